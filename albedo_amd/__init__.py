@@ -1,0 +1,149 @@
+"""albedo_amd: MI355X path-tracing hot path behind a C ABI (libalbedo_gpu.so).
+
+Python mirror of the reference renderer's interface for tests and benchmarks:
+  Context         device context (ab_context_create)
+  GpuScene        a resident scene (reference Scene after its add* calls)
+  render_image    reference renderImage(RenderJob, PixelBuffer)
+                  (render/render_image.cpp:58-77)
+All compute goes through the HIP library; there is no CPU fallback.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from ._abi import AlbedoError, check, lib
+from .scene import (CameraSettings, CheckerTexture, ConductorMaterial, ConstantTexture, DielectricMaterial,
+                    DiffuseMaterial, MeshGeometry, MeshInstance, MixMaterial, PerlinPattern, PerlinTexture,
+                    PointLight, Scene, SkyLight, Sphere, Transform, TriangleMesh, FLAT, SMOOTH)
+
+__all__ = ["Context", "GpuScene", "AlbedoError", "Scene", "Sphere", "TriangleMesh", "MeshGeometry",
+           "MeshInstance", "Transform", "DiffuseMaterial", "ConductorMaterial", "DielectricMaterial",
+           "MixMaterial", "PerlinPattern", "ConstantTexture", "CheckerTexture", "PerlinTexture", "PointLight",
+           "SkyLight", "CameraSettings", "FLAT", "SMOOTH"]
+
+
+class Context:
+    """A library context on one HIP device."""
+
+    def __init__(self, device=0):
+        self.lib = lib()
+        h = ctypes.c_void_p()
+        check(self.lib.ab_context_create(device, ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            self.lib.ab_context_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, status):
+        check(status, self.handle)
+
+    def scene(self, scene: Scene) -> "GpuScene":
+        return GpuScene(self, scene)
+
+    def set_launch_config(self, block_size=256, paths_in_flight=1 << 21, grid_blocks=0, use_graph=1):
+        c = _abi.LaunchConfig(block_size, paths_in_flight, grid_blocks, use_graph)
+        self.check(self.lib.ab_set_launch_config(self.handle, ctypes.byref(c)))
+
+    def launch_config(self):
+        c = _abi.LaunchConfig()
+        self.check(self.lib.ab_get_launch_config(self.handle, ctypes.byref(c)))
+        return c
+
+
+class GpuScene:
+    """A scene resident in HBM (ab_scene_create)."""
+
+    def __init__(self, ctx: Context, scene: Scene):
+        self.ctx = ctx
+        self.desc = scene.to_desc()
+        h = ctypes.c_void_p()
+        ctx.check(ctx.lib.ab_scene_create(ctx.handle, self.desc.ptr(), ctypes.byref(h)))
+        self.handle = h
+
+    def release(self):
+        if self.handle:
+            self.ctx.lib.ab_scene_release(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+    @property
+    def hash(self):
+        return self.ctx.lib.ab_scene_hash(self.handle)
+
+    @property
+    def device_bytes(self):
+        return self.ctx.lib.ab_scene_device_bytes(self.handle)
+
+    def info(self):
+        d = ctypes.c_uint32()
+        t, b, n = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        self.ctx.check(self.ctx.lib.ab_scene_info(self.handle, ctypes.byref(d), ctypes.byref(t), ctypes.byref(b),
+                                                  ctypes.byref(n)))
+        return {"stack_depth": d.value, "tlas_nodes": t.value, "blas_nodes": b.value, "triangles": n.value}
+
+    # ------------------------------------------------------------ ray queries
+
+    def trace_closest(self, rays: np.ndarray) -> np.ndarray:
+        """Host rays [n, 8] float32 -> hits [n, 4] uint32 (t bits, object, prim, bary)."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        hits = np.zeros((len(rays), 4), dtype=np.uint32)
+        self.ctx.check(self.ctx.lib.ab_trace_closest_host(self.handle, rays.ctypes.data, hits.ctypes.data,
+                                                          len(rays)))
+        return hits
+
+    def trace_any(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        occ = np.zeros(len(rays), dtype=np.uint8)
+        self.ctx.check(self.ctx.lib.ab_trace_any_host(self.handle, rays.ctypes.data, occ.ctypes.data, len(rays)))
+        return occ
+
+    def trace_closest_device(self, d_rays: int, d_hits: int, n: int, stream: int = 0):
+        """Device pointers (e.g. torch tensor data_ptr()) on a hipStream_t."""
+        self.ctx.check(self.ctx.lib.ab_trace_closest(self.handle, ctypes.c_void_p(d_rays), ctypes.c_void_p(d_hits),
+                                                     n, ctypes.c_void_p(stream)))
+
+    def trace_any_device(self, d_rays: int, d_occ: int, n: int, stream: int = 0):
+        self.ctx.check(self.ctx.lib.ab_trace_any(self.handle, ctypes.c_void_p(d_rays), ctypes.c_void_p(d_occ), n,
+                                                 ctypes.c_void_p(stream)))
+
+    # -------------------------------------------------------------- rendering
+
+    def render(self, camera: CameraSettings, width, height, samples_per_axis, max_depth=64, roulette_start=3,
+               seed=0, stats=False):
+        """Mean radiance per pixel, float32 [height, width, 3]."""
+        from .workloads import render_params
+        p = render_params(width, height, samples_per_axis, max_depth, roulette_start, seed)
+        cam = camera.to_abi()
+        out = np.zeros((height, width, 3), dtype=np.float32)
+        st = _abi.RenderStats()
+        self.ctx.check(self.ctx.lib.ab_render(self.handle, ctypes.byref(cam), ctypes.byref(p), out.ctypes.data,
+                                              ctypes.byref(st)))
+        if stats:
+            return out, {"camera_rays": st.camera_rays, "extension_rays": st.extension_rays,
+                         "shadow_rays": st.shadow_rays, "paths": st.paths, "gpu_ms": st.gpu_ms}
+        return out
+
+
+def hits_t(hits: np.ndarray) -> np.ndarray:
+    """The t column of a hits array as float32."""
+    return hits[:, 0].copy().view(np.float32)

@@ -1,0 +1,381 @@
+// Device-side ray queries: two-level BVH2 traversal with an LDS stack, the FP32
+// watertight triangle test and the FP64 sphere test.
+//
+// Contraction is off in every function here (#pragma clang fp contract(off)):
+// the Woop edge functions must round each product on its own so shared edges
+// get exactly negated values (reference geometry/triangle.cpp:115-122, built
+// with -ffp-contract=off, src/CMakeLists.txt:118-124), and the instance
+// transform must match the FP32 restatement bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "layout.hpp"
+
+namespace ab {
+
+constexpr uint32_t kMissId = 0xffffffffu;
+
+struct RayF {
+    float ox, oy, oz, tmin;
+    float dx, dy, dz, tmax;
+};
+
+struct HitF {
+    float t;
+    uint32_t object;
+    uint32_t prim;
+    float b1, b2;
+};
+
+__device__ __forceinline__ bool betterHit(float t, uint32_t obj, uint32_t prim, const HitF& h)
+{
+    if (h.object == kMissId) return true;
+    if (t != h.t) return t < h.t;
+    if (obj != h.object) return obj < h.object;
+    return prim < h.prim;
+}
+
+// Sphere test in FP64 (reference geometry/sphere.cpp:92-135): Haines'
+// discriminant a (r^2 - |l|^2) and the stable roots q/a, c/q.
+__device__ __forceinline__ bool sphereHit(const double* s, const RayF& r, double& t_out)
+{
+#pragma clang fp contract(off)
+    const double ox = double(r.ox) - s[0], oy = double(r.oy) - s[1], oz = double(r.oz) - s[2];
+    const double dx = r.dx, dy = r.dy, dz = r.dz;
+    const double a = dx * dx + dy * dy + dz * dz;
+    if (!(a >= 2.2250738585072014e-308 && a <= 1.7976931348623157e308)) return false;
+    const double h = dx * ox + dy * oy + dz * oz;
+    const double rad = s[3];
+    const double c = (ox * ox + oy * oy + oz * oz) - rad * rad;
+    const double ha = h / a;
+    const double lx = ox - dx * ha, ly = oy - dy * ha, lz = oz - dz * ha;
+    const double disc = a * (rad * rad - (lx * lx + ly * ly + lz * lz));
+    if (disc < 0) return false;
+    const double q = -(h + copysign(sqrt(disc), h));
+    double tn = 0, tf = 0;
+    if (q != 0) {
+        tn = q / a;
+        tf = c / q;
+        if (tn > tf) {
+            const double tmp = tn;
+            tn = tf;
+            tf = tmp;
+        }
+    }
+    const double tmin = r.tmin, tmax = r.tmax;
+    double t = tn;
+    if (!(tmin < t && t < tmax)) {
+        t = tf;
+        if (!(tmin < t && t < tmax)) return false;
+    }
+    // The reference forms p = o + d t first, then p - C (sphere.cpp:120-122).
+    const double qx = (double(r.ox) + dx * t) - s[0], qy = (double(r.oy) + dy * t) - s[1],
+                 qz = (double(r.oz) + dz * t) - s[2];
+    if (!(qx * qx + qy * qy + qz * qz > 0)) return false;
+    t_out = t;
+    return true;
+}
+
+// The ray in the sheared space of Woop et al. 2013, set up once per (object
+// space) ray: permutation (kx, ky, kz) and shear (sx, sy, sz).
+struct ShearRay {
+    float ox, oy, oz; // origin permuted to (kx, ky, kz)
+    float sx, sy, sz;
+    int kx, ky, kz;
+    bool valid;       // |d|^2 is a normal number
+};
+
+__device__ __forceinline__ float pick(float x, float y, float z, int k)
+{
+    return k == 0 ? x : (k == 1 ? y : z);
+}
+
+__device__ __forceinline__ ShearRay makeShear(float ox, float oy, float oz, float dx, float dy, float dz)
+{
+#pragma clang fp contract(off)
+    ShearRay s;
+    const float l2 = (dx * dx + dy * dy) + dz * dz;
+    s.valid = l2 >= 1.17549435e-38f && l2 <= 3.40282347e38f;
+    const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+    int kz;
+    if (ax > ay) kz = ax > az ? 0 : 2;
+    else kz = ay > az ? 1 : 2;
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    const float dzv = pick(dx, dy, dz, kz);
+    if (dzv < 0) {
+        const int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    s.kx = kx;
+    s.ky = ky;
+    s.kz = kz;
+    s.sx = pick(dx, dy, dz, kx) / dzv;
+    s.sy = pick(dx, dy, dz, ky) / dzv;
+    s.sz = 1.0f / dzv;
+    s.ox = pick(ox, oy, oz, kx);
+    s.oy = pick(ox, oy, oz, ky);
+    s.oz = pick(ox, oy, oz, kz);
+    return s;
+}
+
+// Watertight triangle test (reference geometry/triangle.cpp:132-176) in FP32,
+// with the edge functions recomputed in FP64 when one is exactly zero.
+__device__ __forceinline__ bool triHit(const ShearRay& s, const float4& v0, const float4& v1, const float4& v2,
+                                       float tmin, float tmax, float& t_out, float& b1_out, float& b2_out)
+{
+#pragma clang fp contract(off)
+    const float az_ = pick(v0.x, v0.y, v0.z, s.kz) - s.oz;
+    const float bz_ = pick(v1.x, v1.y, v1.z, s.kz) - s.oz;
+    const float cz_ = pick(v2.x, v2.y, v2.z, s.kz) - s.oz;
+    const float ax = (pick(v0.x, v0.y, v0.z, s.kx) - s.ox) - s.sx * az_;
+    const float ay = (pick(v0.x, v0.y, v0.z, s.ky) - s.oy) - s.sy * az_;
+    const float bx = (pick(v1.x, v1.y, v1.z, s.kx) - s.ox) - s.sx * bz_;
+    const float by = (pick(v1.x, v1.y, v1.z, s.ky) - s.oy) - s.sy * bz_;
+    const float cx = (pick(v2.x, v2.y, v2.z, s.kx) - s.ox) - s.sx * cz_;
+    const float cy = (pick(v2.x, v2.y, v2.z, s.ky) - s.oy) - s.sy * cz_;
+    float u = cx * by - cy * bx;
+    float v = ax * cy - ay * cx;
+    float w = bx * ay - by * ax;
+    if (u == 0.0f || v == 0.0f || w == 0.0f) {
+        u = float(double(cx) * double(by) - double(cy) * double(bx));
+        v = float(double(ax) * double(cy) - double(ay) * double(cx));
+        w = float(double(bx) * double(ay) - double(by) * double(ax));
+    }
+    const bool inside = (u >= 0.0f && v >= 0.0f && w >= 0.0f) || (u <= 0.0f && v <= 0.0f && w <= 0.0f);
+    if (!inside) return false;
+    const float det = (u + v) + w;
+    if (det == 0.0f) return false;
+    const float az = s.sz * az_, bz = s.sz * bz_, cz = s.sz * cz_;
+    const float t = ((u * az + v * bz) + w * cz) / det;
+    if (!(tmin < t && t < tmax)) return false;
+    t_out = t;
+    b1_out = v / det;
+    b2_out = w / det;
+    return true;
+}
+
+// Slab test of a node's two children (Ize 2013: exits widened by 1 + 2^-20, more
+// than the 1 + 2 gamma(3) rounding bound). A zero direction component is
+// replaced by +-1e-20 so no 0 * inf NaN can arise; the test stays conservative.
+struct BoxRay {
+    float ox, oy, oz, ix, iy, iz;
+};
+
+__device__ __forceinline__ BoxRay makeBoxRay(float ox, float oy, float oz, float dx, float dy, float dz)
+{
+    BoxRay b;
+    b.ox = ox;
+    b.oy = oy;
+    b.oz = oz;
+    b.ix = 1.0f / (fabsf(dx) > 1e-20f ? dx : copysignf(1e-20f, dx));
+    b.iy = 1.0f / (fabsf(dy) > 1e-20f ? dy : copysignf(1e-20f, dy));
+    b.iz = 1.0f / (fabsf(dz) > 1e-20f ? dz : copysignf(1e-20f, dz));
+    return b;
+}
+
+__device__ __forceinline__ float widenFar(float f)
+{
+    return f * (f < 0.0f ? (1.0f - 9.5367431640625e-07f) : (1.0f + 9.5367431640625e-07f));
+}
+
+__device__ __forceinline__ void slab(float lo, float hi, float o, float inv, float& tn, float& tf)
+{
+#pragma clang fp contract(off)
+    const float t0 = (lo - o) * inv, t1 = (hi - o) * inv;
+    tn = fmaxf(tn, fminf(t0, t1));
+    tf = fminf(tf, widenFar(fmaxf(t0, t1)));
+}
+
+__device__ __forceinline__ void testChildren(const DevNode& n, const BoxRay& r, float tlo, float thi, bool& h0,
+                                             bool& h1, float& n0, float& n1)
+{
+    float tn0 = tlo, tf0 = thi, tn1 = tlo, tf1 = thi;
+    slab(n.a[0], n.a[1], r.ox, r.ix, tn0, tf0);
+    slab(n.a[2], n.a[3], r.oy, r.iy, tn0, tf0);
+    slab(n.b[0], n.b[1], r.oz, r.iz, tn0, tf0);
+    slab(n.b[2], n.b[3], r.ox, r.ix, tn1, tf1);
+    slab(n.c[0], n.c[1], r.oy, r.iy, tn1, tf1);
+    slab(n.c[2], n.c[3], r.oz, r.iz, tn1, tf1);
+    h0 = tn0 <= tf0;
+    h1 = tn1 <= tf1 && n.child[1] != kNoChild;
+    n0 = tn0;
+    n1 = tn1;
+}
+
+__device__ __forceinline__ DevNode loadNode(const DevNode* nodes, int32_t i)
+{
+    const float4* p = reinterpret_cast<const float4*>(nodes + i);
+    const float4 a = p[0], b = p[1], c = p[2];
+    const int4 m = reinterpret_cast<const int4*>(p)[3];
+    DevNode n;
+    n.a[0] = a.x; n.a[1] = a.y; n.a[2] = a.z; n.a[3] = a.w;
+    n.b[0] = b.x; n.b[1] = b.y; n.b[2] = b.z; n.b[3] = b.w;
+    n.c[0] = c.x; n.c[1] = c.y; n.c[2] = c.z; n.c[3] = c.w;
+    n.child[0] = m.x;
+    n.child[1] = m.y;
+    return n;
+}
+
+// Upper end of the t range used to cull boxes: a little above the best hit so a
+// triangle whose rounded t ties or beats it is never culled by box rounding.
+__device__ __forceinline__ float cullLimit(float t)
+{
+    return t * (1.0f + 3.814697265625e-06f);
+}
+
+// LDS stack: entry k of this thread lives at stack[k * BLOCK].
+template <int BLOCK>
+struct LdsStack {
+    int32_t* base;
+    __device__ __forceinline__ void put(int sp, int32_t v) { base[sp * BLOCK] = v; }
+    __device__ __forceinline__ int32_t get(int sp) const { return base[sp * BLOCK]; }
+};
+
+// BLAS traversal of one instance with the ray already in object space.
+// ANY: stop at the first hit. Returns true when ANY found a hit.
+template <int BLOCK, bool ANY>
+__device__ __forceinline__ bool traverseBlas(const DevScene& sc, int32_t root, uint32_t object, const RayF& r,
+                                             HitF& best, LdsStack<BLOCK> stk, int sp0)
+{
+    const ShearRay sh = makeShear(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz);
+    if (!sh.valid) return false;
+    const BoxRay br = makeBoxRay(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz);
+    int sp = sp0;
+    int32_t ref = root;
+    while (true) {
+        if (ref >= 0) {
+            const DevNode n = loadNode(sc.nodes, ref);
+            bool h0, h1;
+            float t0, t1;
+            const float lim = ANY ? cullLimit(r.tmax) : cullLimit(best.object == kMissId ? r.tmax : best.t);
+            testChildren(n, br, r.tmin, lim, h0, h1, t0, t1);
+            if (h0 && h1) {
+                const bool swap = t1 < t0;
+                stk.put(sp++, swap ? n.child[0] : n.child[1]);
+                ref = swap ? n.child[1] : n.child[0];
+                continue;
+            }
+            if (h0 || h1) {
+                ref = h0 ? n.child[0] : n.child[1];
+                continue;
+            }
+        } else {
+            const uint32_t enc = uint32_t(~ref);
+            const uint32_t first = enc >> kLeafCountBits, count = enc & ((1u << kLeafCountBits) - 1);
+            for (uint32_t i = 0; i < count; ++i) {
+                const float4* tp = reinterpret_cast<const float4*>(sc.tris + first + i);
+                const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
+                float t, b1, b2;
+                if (triHit(sh, v0, v1, v2, r.tmin, r.tmax, t, b1, b2)) {
+                    if (ANY) return true;
+                    const uint32_t prim = __float_as_uint(v0.w);
+                    if (betterHit(t, object, prim, best)) {
+                        best.t = t;
+                        best.object = object;
+                        best.prim = prim;
+                        best.b1 = b1;
+                        best.b2 = b2;
+                    }
+                }
+            }
+        }
+        if (sp == sp0) return false;
+        ref = stk.get(--sp);
+    }
+}
+
+// Object-space ray of an instance (reference MeshInstance::intersect,
+// geometry/mesh_instance.cpp:32-36, Transform::apply core/transform.cpp:191-210).
+__device__ __forceinline__ RayF toObject(const float* m, const RayF& r)
+{
+#pragma clang fp contract(off)
+    RayF o;
+    o.ox = ((m[0] * r.ox + m[1] * r.oy) + m[2] * r.oz) + m[3];
+    o.oy = ((m[4] * r.ox + m[5] * r.oy) + m[6] * r.oz) + m[7];
+    o.oz = ((m[8] * r.ox + m[9] * r.oy) + m[10] * r.oz) + m[11];
+    o.dx = (m[0] * r.dx + m[1] * r.dy) + m[2] * r.dz;
+    o.dy = (m[4] * r.dx + m[5] * r.dy) + m[6] * r.dz;
+    o.dz = (m[8] * r.dx + m[9] * r.dy) + m[10] * r.dz;
+    o.tmin = r.tmin;
+    o.tmax = r.tmax;
+    return o;
+}
+
+// Tests one scene object. Returns true when ANY found a hit.
+template <int BLOCK, bool ANY>
+__device__ __forceinline__ bool testObject(const DevScene& sc, uint32_t obj, const RayF& r, HitF& best,
+                                           LdsStack<BLOCK> stk, int sp)
+{
+    const DevObject& o = sc.objects[obj];
+    if (o.kind == kObjSphere) {
+        double t;
+        if (sphereHit(sc.spheres + 4 * size_t(obj), r, t)) {
+            if (ANY) return true;
+            const float tf = float(t);
+            if (betterHit(tf, obj, kMissId, best)) {
+                best.t = tf;
+                best.object = obj;
+                best.prim = kMissId;
+                best.b1 = 0.0f;
+                best.b2 = 0.0f;
+            }
+        }
+        return false;
+    }
+    float m[12];
+    const float4* mp = reinterpret_cast<const float4*>(o.to_object);
+    const float4 m0 = mp[0], m1 = mp[1], m2 = mp[2];
+    m[0] = m0.x; m[1] = m0.y; m[2] = m0.z; m[3] = m0.w;
+    m[4] = m1.x; m[5] = m1.y; m[6] = m1.z; m[7] = m1.w;
+    m[8] = m2.x; m[9] = m2.y; m[10] = m2.z; m[11] = m2.w;
+    const RayF ro = toObject(m, r);
+    return traverseBlas<BLOCK, ANY>(sc, o.blas_root, obj, ro, best, stk, sp);
+}
+
+// Full two-level query. ANY: returns true at the first hit in (tmin, tmax);
+// otherwise fills `best` with the lexicographic minimum of (t, object, prim).
+template <int BLOCK, bool ANY>
+__device__ __forceinline__ bool traceRay(const DevScene& sc, const RayF& r, HitF& best, LdsStack<BLOCK> stk)
+{
+    best.t = r.tmax;
+    best.object = kMissId;
+    best.prim = kMissId;
+    best.b1 = best.b2 = 0.0f;
+    if (sc.tlas_root == kNoChild) return false;
+    const BoxRay br = makeBoxRay(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz);
+    int sp = 0;
+    int32_t ref = sc.tlas_root;
+    while (true) {
+        if (ref >= 0) {
+            const DevNode n = loadNode(sc.nodes, ref);
+            bool h0, h1;
+            float t0, t1;
+            const float lim = ANY ? cullLimit(r.tmax) : cullLimit(best.object == kMissId ? r.tmax : best.t);
+            testChildren(n, br, r.tmin, lim, h0, h1, t0, t1);
+            if (h0 && h1) {
+                const bool swap = t1 < t0;
+                stk.put(sp++, swap ? n.child[0] : n.child[1]);
+                ref = swap ? n.child[1] : n.child[0];
+                continue;
+            }
+            if (h0 || h1) {
+                ref = h0 ? n.child[0] : n.child[1];
+                continue;
+            }
+        } else {
+            const uint32_t enc = uint32_t(~ref);
+            const uint32_t first = enc >> kLeafCountBits, count = enc & ((1u << kLeafCountBits) - 1);
+            for (uint32_t i = 0; i < count; ++i) {
+                const uint32_t obj = sc.leafobj[first + i];
+                if (testObject<BLOCK, ANY>(sc, obj, r, best, stk, sp)) return true;
+            }
+        }
+        if (sp == 0) return false;
+        ref = stk.get(--sp);
+    }
+}
+
+} // namespace ab
