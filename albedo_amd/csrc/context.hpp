@@ -28,6 +28,7 @@ struct ab_scene {
 
 namespace ab {
 struct Server;
+class Engine;
 }
 
 struct ab_context {
@@ -38,6 +39,8 @@ struct ab_context {
     ab_launch_config config{256, 1u << 21, 0, 1};
     std::chrono::steady_clock::time_point t0{std::chrono::steady_clock::now()};
     ab::Server* server{nullptr};             // serving loop, when started
+    std::mutex render_mu;                    // serialises ab_render / ab_render_batch
+    ab::Engine* engine{nullptr};             // engine of ab_render / ab_render_batch
     ~ab_context();
 };
 

@@ -1,0 +1,365 @@
+// Wavefront engine host code and the rendering entry points of the C ABI.
+
+#include "engine.hpp"
+
+#include "kernels.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+namespace ab {
+
+namespace {
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+bool finite3(const double* p) { return std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]); }
+
+// Largest max_depth whose draws fit one sample's stride of the pixel stream:
+// 2 jitter draws + at most 7 per bounce.
+constexpr uint32_t kMaxDepthLimit = uint32_t((1024 - 2) / 7);
+
+#define HIP_TRY(expr, what)                                   \
+    do {                                                      \
+        const hipError_t e_ = (expr);                         \
+        if (e_ != hipSuccess) return hipFail(e_, what);       \
+    } while (0)
+
+} // namespace
+
+ab_status prepareJob(const JobSpec& spec, DevJob& dj)
+{
+    const ab_render_params& p = spec.params;
+    const ab_camera& c = spec.camera;
+    if (!spec.scene) return fail(AB_INVALID_ARGUMENT, "render job has no scene");
+    if (p.samples_per_axis == 0)
+        return fail(AB_INVALID_ARGUMENT, "samples per axis must be positive, got 0");
+    if (p.samples_per_axis > 256)
+        return fail(AB_INVALID_ARGUMENT, "samples per axis must be at most 256 (65536 samples per pixel)");
+    if (p.width == 0 || p.height == 0 || uint64_t(p.width) * p.height > (uint64_t(1) << 28))
+        return fail(AB_INVALID_ARGUMENT, "invalid image size " + std::to_string(p.width) + "x" +
+                                             std::to_string(p.height));
+    if (p.max_depth > kMaxDepthLimit)
+        return fail(AB_INVALID_ARGUMENT, "max_depth must be at most " + std::to_string(kMaxDepthLimit));
+    // Camera checks, reference camera/perspective_camera.cpp:24-110.
+    const double fov = c.vfov_degrees;
+    if (std::isnan(fov) || fov <= 0 || fov >= 180)
+        return fail(AB_INVALID_ARGUMENT, "vertical field of view must be in (0, 180) degrees");
+    if (!(c.aspect > 0) || !std::isnormal(c.aspect))
+        return fail(AB_INVALID_ARGUMENT, "aspect ratio must be a positive normal number");
+    if (!finite3(c.look_from) || !finite3(c.look_at) || !finite3(c.up))
+        return fail(AB_INVALID_ARGUMENT, "camera position and target must be finite");
+    double back[3], up[3];
+    for (int k = 0; k < 3; ++k) {
+        back[k] = c.look_from[k] - c.look_at[k];
+        up[k] = c.up[k];
+    }
+    const double bl2 = back[0] * back[0] + back[1] * back[1] + back[2] * back[2];
+    const double ul2 = up[0] * up[0] + up[1] * up[1] + up[2] * up[2];
+    if (!std::isnormal(bl2))
+        return fail(AB_INVALID_ARGUMENT, "distance from camera position to target must be non-zero and finite");
+    if (!std::isnormal(ul2)) return fail(AB_INVALID_ARGUMENT, "camera up direction must be non-zero and finite");
+    double w[3], un[3];
+    for (int k = 0; k < 3; ++k) {
+        w[k] = back[k] / std::sqrt(bl2);
+        un[k] = up[k] / std::sqrt(ul2);
+    }
+    double right[3] = {un[1] * w[2] - un[2] * w[1], un[2] * w[0] - un[0] * w[2], un[0] * w[1] - un[1] * w[0]};
+    const double rl2 = right[0] * right[0] + right[1] * right[1] + right[2] * right[2];
+    if (rl2 < 1e-12) return fail(AB_INVALID_ARGUMENT, "camera up direction must not be parallel to the view");
+    double u[3], v[3];
+    for (int k = 0; k < 3; ++k) u[k] = right[k] / std::sqrt(rl2);
+    v[0] = w[1] * u[2] - w[2] * u[1];
+    v[1] = w[2] * u[0] - w[0] * u[2];
+    v[2] = w[0] * u[1] - w[1] * u[0];
+    const double hh = std::tan(fov * 3.14159265358979323846 / 360);
+    const double hw = c.aspect * hh;
+    std::memset(&dj, 0, sizeof(dj));
+    for (int k = 0; k < 3; ++k) {
+        dj.cam_origin[k] = float(c.look_from[k]);
+        dj.cam_forward[k] = float(-w[k]);
+        dj.cam_hw[k] = float(u[k] * hw);
+        dj.cam_hh[k] = float(v[k] * hh);
+    }
+    dj.scene = spec.scene->dev;
+    dj.width = p.width;
+    dj.height = p.height;
+    dj.spa = p.samples_per_axis;
+    dj.max_depth = p.max_depth;
+    dj.roulette_start = p.roulette_start;
+    dj.seed = p.seed;
+    dj.work_count = uint64_t(p.width) * p.height * p.samples_per_axis * p.samples_per_axis;
+    return AB_OK;
+}
+
+Engine::~Engine() { freeAll(); }
+
+void Engine::freeAll()
+{
+    if (graph_) (void)hipGraphExecDestroy(graph_);
+    graph_ = nullptr;
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (arena_) (void)hipFree(arena_);
+    if (h_counters_) (void)hipHostFree(h_counters_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+    arena_ = nullptr;
+    h_counters_ = nullptr;
+    stream_ = nullptr;
+    capacity_ = max_jobs_ = 0;
+}
+
+ab_status Engine::reserve(uint32_t capacity, uint32_t max_jobs, hipStream_t)
+{
+    if (capacity == capacity_ && max_jobs <= max_jobs_ && arena_) return AB_OK;
+    freeAll();
+    HIP_TRY(hipSetDevice(ctx_->device), "hipSetDevice");
+    HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&h_counters_), sizeof(Counters), hipHostMallocDefault),
+            "hipHostMalloc(counters)");
+    const size_t C = capacity;
+    struct Part {
+        void** dst;
+        size_t bytes;
+    };
+    PathState& p = ps_;
+    Part parts[] = {{reinterpret_cast<void**>(&p.org), C * 16},      {reinterpret_cast<void**>(&p.dir), C * 16},
+                    {reinterpret_cast<void**>(&p.thr), C * 16},      {reinterpret_cast<void**>(&p.rad), C * 16},
+                    {reinterpret_cast<void**>(&p.prev), C * 16},     {reinterpret_cast<void**>(&p.rng), C * 8},
+                    {reinterpret_cast<void**>(&p.pixel), C * 4},     {reinterpret_cast<void**>(&p.hit), C * 16},
+                    {reinterpret_cast<void**>(&p.hit_b2), C * 4},    {reinterpret_cast<void**>(&p.key), C * 4},
+                    {reinterpret_cast<void**>(&p.sh_org), C * 16},   {reinterpret_cast<void**>(&p.sh_dir), C * 16},
+                    {reinterpret_cast<void**>(&p.sh_c), C * 16},     {reinterpret_cast<void**>(&p.active), C * 4},
+                    {reinterpret_cast<void**>(&p.binned), C * 4},    {reinterpret_cast<void**>(&p.shadowq), C * 4},
+                    {reinterpret_cast<void**>(&p.termq), C * 4},     {reinterpret_cast<void**>(&p.freeq), C * 4},
+                    {reinterpret_cast<void**>(&p.ctr), sizeof(Counters)},
+                    {reinterpret_cast<void**>(&d_jobs_), sizeof(DevJob) * std::max<uint32_t>(max_jobs, 1)}};
+    size_t total = 0;
+    for (const Part& part : parts) total += align256(part.bytes);
+    HIP_TRY(hipMalloc(&arena_, total), "hipMalloc(path state)");
+    size_t off = 0;
+    for (const Part& part : parts) {
+        *part.dst = static_cast<char*>(arena_) + off;
+        off += align256(part.bytes);
+    }
+    p.capacity = capacity;
+    p.jobs = d_jobs_;
+    capacity_ = capacity;
+    max_jobs_ = std::max<uint32_t>(max_jobs, 1);
+    return AB_OK;
+}
+
+ab_status Engine::bounceOnce(const LaunchShape& shape)
+{
+    if (use_graph_ && graph_) {
+        HIP_TRY(hipGraphLaunch(graph_, stream_), "hipGraphLaunch");
+        return AB_OK;
+    }
+    HIP_TRY(launchBounce(ps_, shape, stream_), "bounce launch");
+    return AB_OK;
+}
+
+ab_status Engine::readCounters(Counters& out)
+{
+    HIP_TRY(hipMemcpyAsync(h_counters_, ps_.ctr, sizeof(Counters), hipMemcpyDeviceToHost, stream_),
+            "hipMemcpy(counters)");
+    HIP_TRY(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    out = *h_counters_;
+    return AB_OK;
+}
+
+ab_status Engine::runUntilDone(const LaunchShape& shape, bool use_graph, Counters& c)
+{
+    use_graph_ = use_graph;
+    if (graph_) {
+        (void)hipGraphExecDestroy(graph_);
+        graph_ = nullptr;
+    }
+    if (use_graph) {
+        hipGraph_t g = nullptr;
+        HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        const hipError_t le = launchBounce(ps_, shape, stream_);
+        const hipError_t ce = hipStreamEndCapture(stream_, &g);
+        if (le != hipSuccess) return hipFail(le, "bounce capture");
+        HIP_TRY(ce, "hipStreamEndCapture");
+        const hipError_t ie = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIP_TRY(ie, "hipGraphInstantiate");
+    }
+    // Bounces between checks grow geometrically: the queue drains in a tail of
+    // short bounces, and each check costs a host round trip.
+    // Every path ends within max_depth + 1 bounces of its generation, and each
+    // bounce generates up to `capacity` paths, so this many bounces always suffice;
+    // running past it means the engine's bookkeeping is broken.
+    uint64_t max_depth = 0;
+    for (const DevJob& j : jobs_) max_depth = std::max<uint64_t>(max_depth, j.max_depth);
+    const uint64_t limit = 4 * (max_depth + 2) * (work_total_ / capacity_ + 2) + 64;
+    uint64_t bounces = 0;
+    uint32_t per_check = 4;
+    while (true) {
+        if (bounces > limit) return fail(AB_DEVICE_ERROR, "render did not finish within its bounce bound");
+        bounces += per_check;
+        for (uint32_t k = 0; k < per_check; ++k) {
+            const ab_status st = bounceOnce(shape);
+            if (st != AB_OK) return st;
+        }
+        const ab_status st = readCounters(c);
+        if (st != AB_OK) return st;
+        if (idle(c)) return AB_OK;
+        per_check = std::min<uint32_t>(per_check * 2, 64);
+    }
+}
+
+ab_status Engine::renderBatch(const std::vector<JobSpec>& specs, const std::vector<float*>& out,
+                              ab_render_stats* stats, const ab_launch_config& cfg)
+{
+    if (specs.empty()) return AB_OK;
+    jobs_.assign(specs.size(), DevJob{});
+    uint32_t stack = 0;
+    uint64_t total = 0;
+    size_t accum_bytes = 0;
+    for (size_t i = 0; i < specs.size(); ++i) {
+        const ab_status st = prepareJob(specs[i], jobs_[i]);
+        if (st != AB_OK) return st;
+        jobs_[i].work_begin = total;
+        total += jobs_[i].work_count;
+        stack = std::max(stack, jobs_[i].scene.stack_depth);
+        accum_bytes += align256(size_t(jobs_[i].width) * jobs_[i].height * 32 + 8);
+    }
+    if (stack > 64) return fail(AB_INVALID_ARGUMENT, "scene BVH too deep for the wavefront engine (stack > 64)");
+    const uint32_t capacity = std::max<uint32_t>(cfg.paths_in_flight, 1024);
+    ab_status st = reserve(capacity, uint32_t(specs.size()), nullptr);
+    if (st != AB_OK) return st;
+    void* acc = nullptr;
+    HIP_TRY(hipMallocAsync(&acc, accum_bytes, stream_), "hipMalloc(accumulators)");
+    HIP_TRY(hipMemsetAsync(acc, 0, accum_bytes, stream_), "hipMemset(accumulators)");
+    size_t off = 0;
+    for (DevJob& j : jobs_) {
+        j.accum = reinterpret_cast<unsigned long long*>(static_cast<char*>(acc) + off);
+        j.done = j.accum + size_t(j.width) * j.height * 4;
+        off += align256(size_t(j.width) * j.height * 32 + 8);
+    }
+    ps_.n_jobs = uint32_t(jobs_.size());
+    work_total_ = total;
+    HIP_TRY(hipMemcpyAsync(d_jobs_, jobs_.data(), sizeof(DevJob) * jobs_.size(), hipMemcpyHostToDevice, stream_),
+            "hipMemcpy(jobs)");
+    LaunchShape shape{cfg.block_size, cfg.grid_blocks, stack <= 32 ? 32u : 64u};
+    if (shape.grid == 0) shape.grid = std::max<uint32_t>(256, std::min<uint32_t>(capacity / shape.block, 8192));
+    if (shape.block == 256 && shape.stack > 32) shape.block = 128;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0), "hipEventCreate");
+    HIP_TRY(hipEventCreate(&e1), "hipEventCreate");
+    HIP_TRY(launchReset(ps_, total, stream_), "reset launch");
+    HIP_TRY(hipEventRecord(e0, stream_), "hipEventRecord");
+    Counters c{};
+    st = runUntilDone(shape, cfg.use_graph != 0, c);
+    (void)hipEventRecord(e1, stream_);
+    std::vector<unsigned long long> host;
+    for (size_t i = 0; st == AB_OK && i < jobs_.size(); ++i) {
+        const DevJob& j = jobs_[i];
+        const size_t npx = size_t(j.width) * j.height;
+        host.resize(npx * 4);
+        const hipError_t e = hipMemcpyAsync(host.data(), j.accum, npx * 32, hipMemcpyDeviceToHost, stream_);
+        if (e != hipSuccess) {
+            st = hipFail(e, "hipMemcpy(image)");
+            break;
+        }
+        (void)hipStreamSynchronize(stream_);
+        float* dst = out[i];
+        for (size_t px = 0; px < npx; ++px) {
+            const double n = double(host[4 * px + 3]);
+            for (int ch = 0; ch < 3; ++ch)
+                dst[3 * px + ch] = n > 0 ? float(double(host[4 * px + ch]) / 268435456.0 / n) : 0.0f;
+        }
+    }
+    (void)hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFreeAsync(acc, stream_);
+    (void)hipStreamSynchronize(stream_);
+    if (st == AB_OK && stats) {
+        stats->camera_rays = c.camera_rays;
+        stats->extension_rays = c.extend_rays - c.camera_rays;
+        stats->shadow_rays = c.shadow_rays;
+        stats->paths = c.paths_done;
+        stats->gpu_ms = ms;
+    }
+    return st;
+}
+
+Engine& contextEngine(ab_context* ctx)
+{
+    if (!ctx->engine) ctx->engine = new Engine(ctx);
+    return *ctx->engine;
+}
+
+} // namespace ab
+
+using namespace ab;
+
+extern "C" {
+
+ab_status ab_render(ab_scene* scene, const ab_camera* camera, const ab_render_params* params, float* rgb_mean,
+                    ab_render_stats* stats)
+{
+    if (!scene || !camera || !params || !rgb_mean) return fail(AB_INVALID_ARGUMENT, "ab_render: null argument");
+    ab_context* ctx = scene->ctx;
+    std::lock_guard<std::mutex> lock(ctx->render_mu);
+    ab_launch_config cfg;
+    {
+        std::lock_guard<std::mutex> l2(ctx->mu);
+        cfg = ctx->config;
+    }
+    JobSpec js{scene, *camera, *params};
+    return contextEngine(ctx).renderBatch({js}, {rgb_mean}, stats, cfg);
+}
+
+ab_status ab_render_batch(ab_context* ctx, const ab_job_desc* jobs, uint32_t n_jobs, float* const* rgb_means,
+                          ab_render_stats* stats)
+{
+    if (!ctx || (n_jobs && (!jobs || !rgb_means))) return fail(AB_INVALID_ARGUMENT, "ab_render_batch: null argument");
+    std::vector<JobSpec> specs;
+    std::vector<float*> outs;
+    for (uint32_t i = 0; i < n_jobs; ++i) {
+        if (!rgb_means[i]) return fail(AB_INVALID_ARGUMENT, "ab_render_batch: null output");
+        if (jobs[i].scene && jobs[i].scene->ctx != ctx)
+            return fail(AB_INVALID_ARGUMENT, "ab_render_batch: scene belongs to another context");
+        specs.push_back({jobs[i].scene, jobs[i].camera, jobs[i].params});
+        outs.push_back(rgb_means[i]);
+    }
+    std::lock_guard<std::mutex> lock(ctx->render_mu);
+    ab_launch_config cfg;
+    {
+        std::lock_guard<std::mutex> l2(ctx->mu);
+        cfg = ctx->config;
+    }
+    return contextEngine(ctx).renderBatch(specs, outs, stats, cfg);
+}
+
+ab_status ab_set_launch_config(ab_context* ctx, const ab_launch_config* c)
+{
+    if (!ctx || !c) return fail(AB_INVALID_ARGUMENT, "ab_set_launch_config: null argument");
+    if (c->block_size != 64 && c->block_size != 128 && c->block_size != 256)
+        return fail(AB_INVALID_ARGUMENT, "block_size must be 64, 128 or 256");
+    if (c->paths_in_flight < 1024 || c->paths_in_flight > (1u << 26))
+        return fail(AB_INVALID_ARGUMENT, "paths_in_flight must be in [1024, 2^26]");
+    if (c->grid_blocks > (1u << 20)) return fail(AB_INVALID_ARGUMENT, "grid_blocks too large");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->config = *c;
+    return AB_OK;
+}
+
+ab_status ab_get_launch_config(const ab_context* ctx, ab_launch_config* c)
+{
+    if (!ctx || !c) return fail(AB_INVALID_ARGUMENT, "ab_get_launch_config: null argument");
+    std::lock_guard<std::mutex> lock(const_cast<ab_context*>(ctx)->mu);
+    *c = ctx->config;
+    return AB_OK;
+}
+
+} // extern "C"

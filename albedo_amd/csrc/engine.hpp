@@ -1,0 +1,71 @@
+// Host side of the wavefront engine: device memory for path state and queues,
+// the job table, the bounce loop (optionally replayed as a hipGraph).
+#pragma once
+
+#include "context.hpp"
+#include "wavefront.hpp"
+
+#include <vector>
+
+namespace ab {
+
+// A validated job with its camera frame, ready for the device table.
+struct JobSpec {
+    ab_scene* scene{nullptr};
+    ab_camera camera{};
+    ab_render_params params{};
+};
+
+// Checks params and camera as the reference does (render_image.cpp:60-67,
+// perspective_camera.cpp:24-110) and fills the FP32 camera frame of `dj`.
+ab_status prepareJob(const JobSpec& spec, DevJob& dj);
+
+class Engine {
+public:
+    explicit Engine(ab_context* ctx) : ctx_(ctx) {}
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    // Allocates path state for `capacity` slots and room for `max_jobs` jobs.
+    ab_status reserve(uint32_t capacity, uint32_t max_jobs, hipStream_t stream);
+
+    // Renders every job to completion together; out[i] receives job i's
+    // per-pixel mean (width*height*3 floats).
+    ab_status renderBatch(const std::vector<JobSpec>& jobs, const std::vector<float*>& out, ab_render_stats* stats,
+                          const ab_launch_config& cfg);
+
+    // Serving interface: jobs are appended while the engine runs.
+    ab_status beginStream(const ab_launch_config& cfg, uint32_t stack);
+    ab_status appendJob(const JobSpec& job, unsigned long long* accum, unsigned long long* done, uint32_t* index);
+    ab_status cancelJob(uint32_t index);
+    ab_status runBounces(uint32_t count);
+    ab_status readCounters(Counters& out);
+    bool idle(const Counters& c) const { return c.work_next == c.work_total && c.n_active == 0; }
+    uint32_t capacity() const { return capacity_; }
+    hipStream_t stream() const { return stream_; }
+
+private:
+    ab_status runUntilDone(const LaunchShape& shape, bool use_graph, Counters& final_counters);
+    ab_status bounceOnce(const LaunchShape& shape);
+    void freeAll();
+
+    ab_context* ctx_;
+    hipStream_t stream_{nullptr};
+    void* arena_{nullptr};
+    uint32_t capacity_{0};
+    uint32_t max_jobs_{0};
+    PathState ps_{};
+    DevJob* d_jobs_{nullptr};
+    Counters* h_counters_{nullptr}; // pinned
+    std::vector<DevJob> jobs_;
+    unsigned long long work_total_{0};
+    LaunchShape shape_{};
+    hipGraphExec_t graph_{nullptr};
+    bool use_graph_{false};
+};
+
+// Lock-protected engine of a context for ab_render / ab_render_batch.
+Engine& contextEngine(ab_context* ctx);
+
+} // namespace ab

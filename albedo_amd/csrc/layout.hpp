@@ -66,12 +66,13 @@ struct alignas(16) DevObject {
 
 // Per kept triangle data needed only by shading (not by traversal).
 struct alignas(16) DevTriShading {
+    float p0[3], p1[3], p2[3]; // vertices (object space)
     float n0[3], n1[3], n2[3]; // corner normals (object space); face normal if flat
     float fn[3];               // unit face normal (object space)
     float uv[6];
     float pad[2];
 };
-static_assert(sizeof(DevTriShading) == 80, "shading record size");
+static_assert(sizeof(DevTriShading) == 116 + 12, "shading record size");
 
 struct alignas(16) DevMaterial {
     uint32_t kind;     // ab_material_kind

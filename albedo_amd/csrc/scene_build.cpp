@@ -470,6 +470,11 @@ ab_status buildHostScene(const ab_scene_desc& d, HostScene& out, std::string& er
                 e2[k] = p[2][k] - p[0][k];
             }
             cross3(e1, e2, fn);
+            for (int k = 0; k < 3; ++k) {
+                sh.p0[k] = m.positions[3 * ix[0] + k];
+                sh.p1[k] = m.positions[3 * ix[1] + k];
+                sh.p2[k] = m.positions[3 * ix[2] + k];
+            }
             const double fl = std::sqrt(dot3(fn, fn));
             for (int k = 0; k < 3; ++k) sh.fn[k] = float(fn[k] / fl);
             float* cn[3] = {sh.n0, sh.n1, sh.n2};

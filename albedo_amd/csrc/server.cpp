@@ -1,5 +1,6 @@
 // Serving loop (filled in with the batched engine).
 #include "context.hpp"
+#include "engine.hpp"
 
 namespace ab {
 struct Server {};
@@ -8,6 +9,7 @@ struct Server {};
 ab_context::~ab_context()
 {
     delete server;
+    delete engine;
     for (auto& kv : resident) {
         (void)hipFree(kv.second->arena);
         delete kv.second;

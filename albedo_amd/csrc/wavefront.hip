@@ -1,0 +1,612 @@
+// Wavefront path tracer kernels (reference estimator/path_estimator.cpp:73-185
+// and render/render_image.cpp:27-44, rebuilt as generate -> extend -> bin ->
+// shade -> shadow -> finish over queues in HBM).
+//
+// Every path is identified by (job, pixel, sample) and draws from its own PCG32
+// stream position, so its result does not depend on which slot, wave or launch
+// handles it. Radiance is accumulated per pixel as 2^-28 fixed point in 64-bit
+// integers, whose sums do not depend on the order of the atomic adds: images are
+// bit-identical for every launch configuration.
+
+#include "kernels.hpp"
+#include "shade.hpp"
+#include "traverse.hpp"
+#include "wavefront.hpp"
+
+namespace ab {
+
+namespace {
+
+constexpr double kFixedScale = 268435456.0; // 2^28
+constexpr float kMaxSampleRadiance = 1e5f;  // per-sample clamp that keeps 64-bit sums exact
+
+// Position of this lane in a wave-aggregated append to *counter (lanes with pred).
+__device__ __forceinline__ uint32_t waveAppend(uint32_t* counter, bool pred)
+{
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0) return 0;
+    const int lane = __lane_id();
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, uint32_t(__popcll(mask)));
+    base = __shfl(base, leader);
+    return base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+}
+
+__device__ __forceinline__ uint32_t findJob(const DevJob* jobs, uint32_t n, unsigned long long w)
+{
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (jobs[mid].work_begin <= w) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ float powerHeuristic(float chosen, float other)
+{
+    const float c2 = chosen * chosen;
+    const float t = c2 + other * other;
+    return t > 0.0f ? c2 / t : 0.0f;
+}
+
+__device__ __forceinline__ uint32_t gridStart() { return blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ __forceinline__ uint32_t gridStride() { return gridDim.x * blockDim.x; }
+
+// ---------------------------------------------------------------- generate
+
+// Camera rays for the next n_new samples (reference render_image.cpp:27-44 and
+// PerspectiveCamera::generateRay, perspective_camera.cpp:74-87). Work index w of
+// a job maps to sample s = w / (W H) and pixel w mod (W H), so every pixel gets
+// its first sample before any gets its second.
+__global__ void kGenerate(PathState ps)
+{
+    const Counters* c = ps.ctr;
+    const uint32_t n_new = c->n_new, free_top = c->n_free, act0 = c->n_active;
+    const unsigned long long w0 = c->work_next;
+    for (uint32_t i = gridStart(); i < n_new; i += gridStride()) {
+        const uint32_t slot = ps.freeq[free_top - 1 - i];
+        const unsigned long long w = w0 + i;
+        const uint32_t j = findJob(ps.jobs, ps.n_jobs, w);
+        const DevJob& job = ps.jobs[j];
+        const unsigned long long local = w - job.work_begin;
+        const unsigned long long npx = uint64_t(job.width) * job.height;
+        const uint32_t s = uint32_t(local / npx);
+        const uint32_t pix = uint32_t(local - uint64_t(s) * npx);
+        const uint32_t x = pix % job.width, y = pix / job.width;
+        Rng rng = sampleRng(job.seed, pix, s);
+        const uint32_t sx = s % job.spa, sy = s / job.spa;
+        const float jx = rng.uniform(), jy = rng.uniform();
+        const float n = float(job.spa);
+        const float fx = (float(x) + (float(sx) + jx) / n) / float(job.width);
+        const float fy = (float(y) + (float(sy) + jy) / n) / float(job.height);
+        const float h = 2.0f * fx - 1.0f, v = 1.0f - 2.0f * fy;
+        const F3 fw = f3(job.cam_forward[0], job.cam_forward[1], job.cam_forward[2]);
+        const F3 hw = f3(job.cam_hw[0], job.cam_hw[1], job.cam_hw[2]);
+        const F3 hh = f3(job.cam_hh[0], job.cam_hh[1], job.cam_hh[2]);
+        const F3 d = normalize(fw + hw * h + hh * v);
+        const float ox = job.cam_origin[0], oy = job.cam_origin[1], oz = job.cam_origin[2];
+        ps.org[slot] = make_float4(ox, oy, oz, 0.0f);
+        ps.dir[slot] = make_float4(d.x, d.y, d.z, 1.0f);
+        ps.thr[slot] = make_float4(1.0f, 1.0f, 1.0f, __int_as_float(0));
+        ps.rad[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        ps.prev[slot] = make_float4(ox, oy, oz, __uint_as_float(j));
+        ps.rng[slot] = rng.state;
+        ps.pixel[slot] = pix;
+        ps.active[act0 + i] = slot;
+    }
+}
+
+__global__ void kPostGenerate(PathState ps)
+{
+    Counters* c = ps.ctr;
+    const uint32_t n = c->n_new;
+    c->n_active += n;
+    c->n_free -= n;
+    c->work_next += n;
+    c->camera_rays += n;
+    c->n_new = 0;
+    for (int b = 0; b < 8; ++b) c->bin_count[b] = 0;
+}
+
+// ------------------------------------------------------------------ extend
+
+__device__ __forceinline__ uint32_t materialBin(const DevScene& sc, uint32_t obj)
+{
+    if (obj == kMissId) return 0;
+    return sc.materials[sc.objects[obj].material].kind + 1;
+}
+
+template <int BLOCK, int STACK>
+__global__ __launch_bounds__(BLOCK) void kExtend(PathState ps)
+{
+    __shared__ int32_t stack[STACK * BLOCK];
+    Counters* c = ps.ctr;
+    const uint32_t n = c->n_active;
+    for (uint32_t i = gridStart(); i < n; i += gridStride()) {
+        const uint32_t slot = ps.active[i];
+        const float4 o = ps.org[slot], d = ps.dir[slot];
+        const uint32_t j = __float_as_uint(ps.prev[slot].w);
+        const DevScene& sc = ps.jobs[j].scene;
+        const RayF r{o.x, o.y, o.z, 0.0f, d.x, d.y, d.z, __builtin_huge_valf()};
+        HitF h;
+        traceRay<BLOCK, false>(sc, r, h, LdsStack<BLOCK>{stack + threadIdx.x});
+        ps.hit[slot] = make_uint4(__float_as_uint(h.t), h.object, h.prim, __float_as_uint(h.b1));
+        ps.hit_b2[slot] = h.b2;
+        const uint32_t key = materialBin(sc, h.object);
+        ps.key[slot] = key;
+        for (uint32_t b = 0; b < uint32_t(kBins); ++b) {
+            const unsigned long long m = __ballot(key == b);
+            if (m && __lane_id() == __ffsll(static_cast<long long>(m)) - 1)
+                atomicAdd(&c->bin_count[b], uint32_t(__popcll(m)));
+        }
+    }
+}
+
+__global__ void kBinPlan(PathState ps)
+{
+    Counters* c = ps.ctr;
+    uint32_t off = 0;
+    for (int b = 0; b < kBins; ++b) {
+        c->bin_off[b] = off;
+        c->bin_fill[b] = 0;
+        off += c->bin_count[b];
+    }
+    c->extend_rays += c->n_active;
+}
+
+// Counting sort of the extended paths by material, so waves of the shade kernel
+// mostly run one material's code.
+__global__ void kScatter(PathState ps)
+{
+    Counters* c = ps.ctr;
+    const uint32_t n = c->n_active;
+    for (uint32_t i = gridStart(); i < n; i += gridStride()) {
+        const uint32_t slot = ps.active[i];
+        const uint32_t key = ps.key[slot];
+        uint32_t pos = 0;
+        for (uint32_t b = 0; b < uint32_t(kBins); ++b) {
+            const uint32_t p = waveAppend(&c->bin_fill[b], key == b);
+            if (key == b) pos = c->bin_off[b] + p;
+        }
+        ps.binned[pos] = slot;
+    }
+}
+
+__global__ void kPrepShade(PathState ps)
+{
+    Counters* c = ps.ctr;
+    c->n_in = c->n_active;
+    c->n_active = 0;
+    c->n_shadow = 0;
+    c->n_term = 0;
+}
+
+// ------------------------------------------------------------------- shade
+
+struct SurfaceHit {
+    F3 p, n, ns; // point, geometric and shading normals facing the ray
+    bool front;
+    float u, v;
+};
+
+__device__ __forceinline__ SurfaceHit surfaceAt(const DevScene& sc, const uint4& h, float b2, F3 ro, F3 rd,
+                                                bool want_uv)
+{
+    SurfaceHit s;
+    const uint32_t obj = h.y;
+    const DevObject& o = sc.objects[obj];
+    const float t = __uint_as_float(h.x);
+    F3 n, ns;
+    if (o.kind == kObjSphere) {
+        // reference geometry/sphere.cpp:119-134 (point, normal) and :137-147 (uv)
+        const double* sp = sc.spheres + 4 * size_t(obj);
+        const double px = double(ro.x) + double(rd.x) * double(t);
+        const double py = double(ro.y) + double(rd.y) * double(t);
+        const double pz = double(ro.z) + double(rd.z) * double(t);
+        const double ex = px - sp[0], ey = py - sp[1], ez = pz - sp[2];
+        const double il = 1.0 / sqrt(ex * ex + ey * ey + ez * ez);
+        n = f3(float(ex * il), float(ey * il), float(ez * il));
+        ns = n;
+        s.p = f3(float(px), float(py), float(pz));
+        s.u = s.v = 0.0f;
+        if (want_uv) {
+            s.u = (atan2f(-n.z, n.x) + kPiF) / (2.0f * kPiF);
+            s.v = acosf(fminf(fmaxf(-n.y, -1.0f), 1.0f)) / kPiF;
+        }
+    } else {
+        // reference mesh_geometry.cpp:77-98, mesh_instance.cpp:32-60
+        const DevTriShading& tr = sc.shading[o.tri_shading_base + h.z];
+        const float b1 = __uint_as_float(h.w), b0 = 1.0f - b1 - b2;
+        const F3 p0 = f3(tr.p0[0], tr.p0[1], tr.p0[2]), p1 = f3(tr.p1[0], tr.p1[1], tr.p1[2]),
+                 p2 = f3(tr.p2[0], tr.p2[1], tr.p2[2]);
+        const F3 pl = p0 + (p1 - p0) * b1 + (p2 - p0) * b2;
+        const F3 fn = f3(tr.fn[0], tr.fn[1], tr.fn[2]);
+        F3 snl = fn;
+        if (o.smooth) {
+            const F3 bl = f3(tr.n0[0], tr.n0[1], tr.n0[2]) * b0 + f3(tr.n1[0], tr.n1[1], tr.n1[2]) * b1 +
+                          f3(tr.n2[0], tr.n2[1], tr.n2[2]) * b2;
+            if (len2(bl) >= 1e-12f) snl = normalize(bl);
+        }
+        s.u = b0 * tr.uv[0] + b1 * tr.uv[2] + b2 * tr.uv[4];
+        s.v = b0 * tr.uv[1] + b1 * tr.uv[3] + b2 * tr.uv[5];
+        const float* W = o.to_world;
+        const float* I = o.to_object;
+        s.p = f3(W[0] * pl.x + W[1] * pl.y + W[2] * pl.z + W[3], W[4] * pl.x + W[5] * pl.y + W[6] * pl.z + W[7],
+                 W[8] * pl.x + W[9] * pl.y + W[10] * pl.z + W[11]);
+        n = normalize(f3(I[0] * fn.x + I[4] * fn.y + I[8] * fn.z, I[1] * fn.x + I[5] * fn.y + I[9] * fn.z,
+                         I[2] * fn.x + I[6] * fn.y + I[10] * fn.z));
+        if (o.smooth) {
+            const F3 sw = f3(I[0] * snl.x + I[4] * snl.y + I[8] * snl.z, I[1] * snl.x + I[5] * snl.y + I[9] * snl.z,
+                             I[2] * snl.x + I[6] * snl.y + I[10] * snl.z);
+            ns = isNormalF(len2(sw)) ? normalize(sw) : n;
+        } else {
+            ns = n;
+        }
+    }
+    s.front = dot(rd, n) < 0.0f;
+    s.n = s.front ? n : -n;
+    s.ns = s.front ? ns : -ns;
+    return s;
+}
+
+// Cone of an area light's sphere seen from p (reference light/area_light.cpp:31-51).
+struct ConeF {
+    F3 axis;
+    float dist, tangent2, cos_max, omc;
+};
+__device__ __forceinline__ bool coneTowards(const DevLight& l, F3 p, ConeF& c)
+{
+    const F3 tc = f3(float(l.position[0] - double(p.x)), float(l.position[1] - double(p.y)),
+                     float(l.position[2] - double(p.z)));
+    const float d2 = len2(tc);
+    const float r = float(l.radius);
+    const float r2 = r * r;
+    if (d2 <= r2) return false;
+    const float d = sqrtf(d2);
+    const float s2 = r2 / d2;
+    const float cm = sqrtf(1.0f - s2);
+    const float omc = s2 / (1.0f + cm);
+    if (!isNormalF(omc)) return false;
+    c.axis = tc / d;
+    c.dist = d;
+    c.tangent2 = d2 - r2;
+    c.cos_max = cm;
+    c.omc = omc;
+    return true;
+}
+
+// Density of choosing direction `dir` from p by sampling light l (reference
+// light/area_light.cpp:84-94, sky_light.hpp, point_light.hpp).
+__device__ __forceinline__ float lightPdf(const DevLight& l, F3 p, F3 dir)
+{
+    if (l.kind == 0u) return 0.0f;
+    if (l.kind == 2u) return kUniformSpherePdf;
+    ConeF c;
+    if (!coneTowards(l, p, c)) return 0.0f;
+    if (dot(dir, c.axis) < c.cos_max) return 0.0f;
+    return 1.0f / (2.0f * kPiF * c.omc);
+}
+
+__device__ __forceinline__ F3 skyRadiance(const DevLight& l, F3 dir)
+{
+    const float h = 0.5f * (dir.y + 1.0f);
+    return f3(l.a[0], l.a[1], l.a[2]) * (1.0f - h) + f3(l.b[0], l.b[1], l.b[2]) * h;
+}
+
+struct LightSampleF {
+    F3 dir, radiance;
+    float dist, pdf;
+    bool delta;
+};
+
+// reference light/point_light.cpp:7-21, area_light.cpp:53-82, sky_light.cpp:17-24
+__device__ __forceinline__ bool sampleLight(const DevLight& l, F3 p, float ux, float uy, LightSampleF& s)
+{
+    if (l.kind == 0u) {
+        const F3 to = f3(float(l.position[0] - double(p.x)), float(l.position[1] - double(p.y)),
+                         float(l.position[2] - double(p.z)));
+        const float d2 = len2(to);
+        if (d2 == 0.0f) return false;
+        const float d = sqrtf(d2);
+        s.dir = to / d;
+        s.radiance = f3(l.a[0], l.a[1], l.a[2]) / d2;
+        s.dist = d;
+        s.pdf = 1.0f;
+        s.delta = true;
+        return true;
+    }
+    if (l.kind == 2u) {
+        s.dir = uniformSphere(ux, uy);
+        s.radiance = skyRadiance(l, s.dir);
+        s.dist = __builtin_huge_valf();
+        s.pdf = kUniformSpherePdf;
+        s.delta = false;
+        return true;
+    }
+    ConeF c;
+    if (!coneTowards(l, p, c)) return false;
+    const float omc = ux * c.omc;
+    const float ct = 1.0f - omc;
+    const float s2 = omc * (2.0f - omc);
+    const float st = sqrtf(s2);
+    float sn, cs;
+    sincosf(2.0f * kPiF * uy, &sn, &cs);
+    const Frame f = frameFrom(c.axis);
+    s.dir = toWorld(f, f3(st * cs, st * sn, ct));
+    const float r = float(l.radius);
+    const float root = sqrtf(fmaxf(0.0f, r * r - c.dist * c.dist * s2));
+    s.dist = c.tangent2 / (c.dist * ct + root);
+    s.radiance = f3(l.a[0], l.a[1], l.a[2]);
+    s.pdf = 1.0f / (2.0f * kPiF * c.omc);
+    s.delta = false;
+    return true;
+}
+
+// One bounce of PathEstimator::radiance (reference path_estimator.cpp:153-185)
+// for every extended path: emission with MIS, next-event estimation (the shadow
+// ray is deferred to kShadow), BSDF sampling and Russian roulette.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void kShade(PathState ps)
+{
+    Counters* c = ps.ctr;
+    const uint32_t n = c->n_in;
+    for (uint32_t i = gridStart(); i < n; i += gridStride()) {
+        const uint32_t slot = ps.binned[i];
+        const float4 o4 = ps.org[slot], d4 = ps.dir[slot], t4 = ps.thr[slot], r4 = ps.rad[slot],
+                     p4 = ps.prev[slot];
+        const uint32_t j = __float_as_uint(p4.w);
+        const DevJob& job = ps.jobs[j];
+        const DevScene& sc = job.scene;
+        const F3 ro = f3(o4.x, o4.y, o4.z), rd = f3(d4.x, d4.y, d4.z), arr_o = f3(p4.x, p4.y, p4.z);
+        const float arr_pdf = o4.w;
+        const bool arr_spec = d4.w != 0.0f;
+        F3 thr = f3(t4.x, t4.y, t4.z), L = f3(r4.x, r4.y, r4.z);
+        const int depth = __float_as_int(t4.w);
+        const uint4 h = ps.hit[slot];
+        const float nl = float(sc.num_lights);
+        bool alive = false, shadow = false;
+        if (job.cancelled) {
+            // terminated without gathering
+        } else if (h.y == kMissId) {
+            if (sc.sky_light >= 0) {
+                const DevLight& sky = sc.lights[sc.sky_light];
+                F3 e = skyRadiance(sky, rd);
+                if (!isBlack(e) && !arr_spec) e = e * powerHeuristic(arr_pdf, (1.0f / nl) * kUniformSpherePdf);
+                L = L + thr * e;
+            }
+        } else {
+            const DevObject& obj = sc.objects[h.y];
+            const DevMaterial* m = &sc.materials[obj.material];
+            const bool uv_needed = m->kind == 0u || m->kind == 3u;
+            const SurfaceHit s = surfaceAt(sc, h, ps.hit_b2[slot], ro, rd, uv_needed);
+            if (obj.light >= 0 && s.front) {
+                const DevLight& l = sc.lights[obj.light];
+                if (l.kind == 1u) {
+                    F3 e = f3(l.a[0], l.a[1], l.a[2]);
+                    if (!isBlack(e) && !arr_spec)
+                        e = e * powerHeuristic(arr_pdf, (1.0f / nl) * lightPdf(l, arr_o, rd));
+                    L = L + thr * e;
+                }
+            }
+            if (depth < int(job.max_depth)) {
+                Rng rng;
+                rng.state = ps.rng[slot];
+                rng.inc = (mixBits(ps.pixel[slot]) << 1u) | 1u;
+                if (m->kind == 3u) {
+                    const float w = patternWeight(s.p, m->mix_scale, m->mix_veins);
+                    m = &sc.materials[w < m->mix_threshold ? m->first : m->second];
+                }
+                Scatter sct;
+                sct.frame = frameFrom(s.ns);
+                sct.geo = s.n;
+                const float a = m->roughness * m->roughness;
+                if (m->kind == 1u) {
+                    sct.lobe = Lobe{kConductor, f3(m->a[0], m->a[1], m->a[2]), f3(m->b[0], m->b[1], m->b[2]), a, 1.0f};
+                } else if (m->kind == 2u) {
+                    sct.lobe = Lobe{kDielectric, f3(0, 0, 0), f3(0, 0, 0), a, s.front ? m->ior : 1.0f / m->ior};
+                } else {
+                    sct.lobe = Lobe{kLambert, textureAt(*m, s.p, s.u, s.v), f3(0, 0, 0), 0.0f, 1.0f};
+                }
+                const F3 wo = -rd;
+                // Next-event estimation (path_estimator.cpp:73-102): three draws always.
+                const float ul = rng.uniform(), lx = rng.uniform(), ly = rng.uniform();
+                if (sc.num_lights > 0) {
+                    const uint32_t li = min(uint32_t(ul * nl), sc.num_lights - 1);
+                    const DevLight& l = sc.lights[li];
+                    LightSampleF ls;
+                    if (sampleLight(l, s.p, lx, ly, ls) && !isBlack(ls.radiance)) {
+                        const F3 f = sct.eval(wo, ls.dir);
+                        if (!isBlack(f)) {
+                            // Stop short of the light by a relative 1e-3 (at least 1e-3):
+                            // more than the FP32 ray-origin offset moves the distance.
+                            const float tmax = isinf(ls.dist) ? ls.dist : ls.dist - fmaxf(ls.dist * 1e-3f, 1e-3f);
+                            const float lpdf = (1.0f / nl) * ls.pdf;
+                            const float w = ls.delta ? 1.0f : powerHeuristic(lpdf, sct.pdf(wo, ls.dir));
+                            const float ct = fabsf(dot(ls.dir, s.ns));
+                            const F3 contrib = thr * f * ls.radiance * (ct * w / lpdf);
+                            if (tmax > 0.0f) {
+                                const F3 so = offsetRay(s.p, dot(ls.dir, s.n) >= 0.0f ? s.n : -s.n);
+                                ps.sh_org[slot] = make_float4(so.x, so.y, so.z, tmax);
+                                ps.sh_dir[slot] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, 0.0f);
+                                ps.sh_c[slot] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
+                                shadow = true;
+                            } else {
+                                L = L + contrib;
+                            }
+                        }
+                    }
+                }
+                const float uc = rng.uniform(), ux = rng.uniform(), uy = rng.uniform();
+                Sample smp;
+                if (sct.sample(wo, uc, ux, uy, smp)) {
+                    thr = thr * smp.weight;
+                    if (!isBlack(thr)) {
+                        alive = true;
+                        if (depth >= int(job.roulette_start)) {
+                            const float surv = fminf(0.95f, maxc(thr));
+                            if (rng.uniform() >= surv) alive = false;
+                            else thr = thr / surv;
+                        }
+                    }
+                }
+                ps.rng[slot] = rng.state;
+                if (alive) {
+                    const F3 no = offsetRay(s.p, dot(smp.dir, s.n) >= 0.0f ? s.n : -s.n);
+                    ps.org[slot] = make_float4(no.x, no.y, no.z, smp.pdf);
+                    ps.dir[slot] = make_float4(smp.dir.x, smp.dir.y, smp.dir.z, smp.specular ? 1.0f : 0.0f);
+                    ps.thr[slot] = make_float4(thr.x, thr.y, thr.z, __int_as_float(depth + 1));
+                    ps.prev[slot] = make_float4(s.p.x, s.p.y, s.p.z, p4.w);
+                }
+            }
+        }
+        ps.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
+        const uint32_t pa = waveAppend(&c->n_active, alive);
+        if (alive) ps.active[pa] = slot;
+        const uint32_t psh = waveAppend(&c->n_shadow, shadow);
+        if (shadow) ps.shadowq[psh] = slot;
+        const uint32_t pt = waveAppend(&c->n_term, !alive);
+        if (!alive) ps.termq[pt] = slot;
+    }
+}
+
+// ------------------------------------------------------------------ shadow
+
+// Occlusion of the deferred light samples (reference Scene::occluded,
+// scene/scene.cpp:157-165); unoccluded contributions join the path's radiance.
+template <int BLOCK, int STACK>
+__global__ __launch_bounds__(BLOCK) void kShadow(PathState ps)
+{
+    __shared__ int32_t stack[STACK * BLOCK];
+    const uint32_t n = ps.ctr->n_shadow;
+    for (uint32_t i = gridStart(); i < n; i += gridStride()) {
+        const uint32_t slot = ps.shadowq[i];
+        const float4 o = ps.sh_org[slot], d = ps.sh_dir[slot];
+        const uint32_t j = __float_as_uint(ps.prev[slot].w);
+        const RayF r{o.x, o.y, o.z, 0.0f, d.x, d.y, d.z, o.w};
+        HitF h;
+        if (!traceRay<BLOCK, true>(ps.jobs[j].scene, r, h, LdsStack<BLOCK>{stack + threadIdx.x})) {
+            const float4 cc = ps.sh_c[slot];
+            float4 rr = ps.rad[slot];
+            rr.x += cc.x;
+            rr.y += cc.y;
+            rr.z += cc.z;
+            ps.rad[slot] = rr;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ finish
+
+__device__ __forceinline__ unsigned long long toFixed(float v)
+{
+    if (!(v > 0.0f)) return 0ull;
+    return static_cast<unsigned long long>(double(fminf(v, kMaxSampleRadiance)) * kFixedScale + 0.5);
+}
+
+// Adds each finished path's radiance to its pixel (reference PixelBuffer::addSample,
+// image/pixel_buffer.cpp:29-35) and returns its slot to the free list.
+__global__ void kFinish(PathState ps)
+{
+    Counters* c = ps.ctr;
+    const uint32_t n = c->n_term, free0 = c->n_free;
+    for (uint32_t i = gridStart(); i < n; i += gridStride()) {
+        const uint32_t slot = ps.termq[i];
+        const uint32_t j = __float_as_uint(ps.prev[slot].w);
+        const DevJob& job = ps.jobs[j];
+        if (!job.cancelled) {
+            const float4 r = ps.rad[slot];
+            unsigned long long* a = job.accum + 4 * size_t(ps.pixel[slot]);
+            atomicAdd(a + 0, toFixed(r.x));
+            atomicAdd(a + 1, toFixed(r.y));
+            atomicAdd(a + 2, toFixed(r.z));
+            atomicAdd(a + 3, 1ull);
+            atomicAdd(job.done, 1ull);
+        }
+        ps.freeq[free0 + i] = slot;
+    }
+}
+
+__global__ void kPost(PathState ps)
+{
+    Counters* c = ps.ctr;
+    c->n_free += c->n_term;
+    c->paths_done += c->n_term;
+    c->shadow_rays += c->n_shadow;
+    const unsigned long long left = c->work_total - c->work_next;
+    c->n_new = uint32_t(left < c->n_free ? left : c->n_free);
+}
+
+__global__ void kReset(PathState ps, unsigned long long total)
+{
+    const uint32_t i0 = gridStart();
+    for (uint32_t i = i0; i < ps.capacity; i += gridStride()) ps.freeq[i] = i;
+    if (i0 == 0) {
+        Counters* c = ps.ctr;
+        *c = Counters{};
+        c->n_free = ps.capacity;
+        c->work_total = total;
+        c->n_new = uint32_t(total < ps.capacity ? total : ps.capacity);
+    }
+}
+
+__global__ void kAddWork(PathState ps, unsigned long long total)
+{
+    Counters* c = ps.ctr;
+    c->work_total = total;
+    const unsigned long long left = total - c->work_next;
+    c->n_new = uint32_t(left < c->n_free ? left : c->n_free);
+}
+
+template <int BLOCK, int STACK>
+hipError_t bounce(const PathState& ps, uint32_t grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(kGenerate, dim3(grid), dim3(BLOCK), 0, s, ps);
+    hipLaunchKernelGGL(kPostGenerate, dim3(1), dim3(1), 0, s, ps);
+    hipLaunchKernelGGL((kExtend<BLOCK, STACK>), dim3(grid), dim3(BLOCK), 0, s, ps);
+    hipLaunchKernelGGL(kBinPlan, dim3(1), dim3(1), 0, s, ps);
+    hipLaunchKernelGGL(kScatter, dim3(grid), dim3(BLOCK), 0, s, ps);
+    hipLaunchKernelGGL(kPrepShade, dim3(1), dim3(1), 0, s, ps);
+    hipLaunchKernelGGL((kShade<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
+    hipLaunchKernelGGL((kShadow<BLOCK, STACK>), dim3(grid), dim3(BLOCK), 0, s, ps);
+    hipLaunchKernelGGL(kFinish, dim3(grid), dim3(BLOCK), 0, s, ps);
+    hipLaunchKernelGGL(kPost, dim3(1), dim3(1), 0, s, ps);
+    return hipGetLastError();
+}
+
+template <int BLOCK>
+hipError_t bounceB(const PathState& ps, const LaunchShape& sh, hipStream_t s)
+{
+    if (sh.stack <= 32) return bounce<BLOCK, 32>(ps, sh.grid, s);
+    if (sh.stack <= 64) return bounce<BLOCK, 64>(ps, sh.grid, s);
+    return hipErrorInvalidValue;
+}
+
+} // namespace
+
+hipError_t launchBounce(const PathState& ps, const LaunchShape& sh, hipStream_t s)
+{
+    switch (sh.block) {
+    case 64: return bounceB<64>(ps, sh, s);
+    case 128: return bounceB<128>(ps, sh, s);
+    case 256:
+        if (sh.stack <= 32) return bounce<256, 32>(ps, sh.grid, s);
+        return hipErrorInvalidValue;
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launchReset(const PathState& ps, unsigned long long total, hipStream_t s)
+{
+    hipLaunchKernelGGL(kReset, dim3(1024), dim3(256), 0, s, ps, total);
+    return hipGetLastError();
+}
+
+hipError_t launchAddWork(const PathState& ps, unsigned long long total, hipStream_t s)
+{
+    hipLaunchKernelGGL(kAddWork, dim3(1), dim3(1), 0, s, ps, total);
+    return hipGetLastError();
+}
+
+} // namespace ab

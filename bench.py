@@ -1,0 +1,143 @@
+"""Headline benchmark: closest-hit throughput on the trefoil microbenchmark.
+
+Workload: a 512 x 32-segment trefoil tube (32 768 FP32 triangles, binned-SAH
+BVH) and two ray sets of 2^24 rays each, coherent (pinhole camera at distance 8)
+and incoherent (origins on a radius-6 sphere, targets uniform in the mesh box).
+One step traces both sets once with ab_trace_closest, rays and hits resident in
+HBM. value = rays / second over the timed steps, in Mrays/s.
+
+roofline: 48 algorithmic bytes per ray (32 B ray read + 16 B hit written) over
+the closest-hit kernel's average launch time, measured with HIP events on the
+launch stream, against 8 TB/s HBM.
+cpu_baseline: the oracle (the CPU restatement of the reference's FP64
+traversal) on a bounded sample of the same rays, on all host cores.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_RAY = 48
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trace_closest_trefoil_traffic.json")
+
+
+def cpu_baseline(scene, rays_sets, budget_s=12.0):
+    """Oracle FP64 closest hit on a bounded sample, all host cores."""
+    import oracle
+    o = oracle.OracleScene(scene.to_desc())
+    threads = os.cpu_count() or 1
+    # Calibrate on 2^16 evenly spaced rays per set, then size the sample to ~budget_s.
+    t0 = time.perf_counter()
+    for r in rays_sets:
+        o.trace_closest(r[:: len(r) >> 16], fp32=False, threads=threads)
+    rate = 2 * (1 << 16) / (time.perf_counter() - t0)
+    per_set = int(min(len(rays_sets[0]), max(1 << 16, rate * budget_s / 2)))
+    stride = max(1, len(rays_sets[0]) // per_set)
+    done, elapsed = 0, 0.0
+    for r in rays_sets:
+        sample = r[::stride]
+        t0 = time.perf_counter()
+        o.trace_closest(sample, fp32=False, threads=threads)
+        elapsed += time.perf_counter() - t0
+        done += len(sample)
+    return {"value": round(done / elapsed / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": "every %d-th ray of both 2^24 sets (%d rays), FP64 oracle traversal, %.1f s"
+                      % (stride, done, elapsed)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one MI355X: this path has no multi-GPU form (each GPU would trace its own "
+              "share of rays against a replicated scene); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import albedo_amd
+    from albedo_amd import workloads as W
+
+    scene, mesh = W.trefoil_scene()
+    sets = {"coherent": W.coherent_rays(mesh, 4096), "incoherent": W.incoherent_rays(mesh, 1 << 24)}
+    n_per = 1 << 24
+    ctx = albedo_amd.Context(0)
+    g = ctx.scene(scene)
+    dev = torch.device("cuda:0")
+    d_rays = {k: torch.from_numpy(v).to(dev) for k, v in sets.items()}
+    d_hits = {k: torch.empty((n_per, 4), dtype=torch.int32, device=dev) for k in sets}
+    stream = torch.cuda.current_stream(dev)
+    sp = stream.cuda_stream
+
+    def step(events=None):
+        for k in sets:
+            if events is not None:
+                events[k][0].record(stream)
+            g.trace_closest_device(d_rays[k].data_ptr(), d_hits[k].data_ptr(), n_per, sp)
+            if events is not None:
+                events[k][1].record(stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    per_set = {k: [] for k in sets}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ev = {k: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for k in sets}
+        step(ev)
+        for k in sets:
+            per_set[k].append(ev[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = {k: float(np.mean([a.elapsed_time(b) for a, b in v])) for k, v in per_set.items()}
+    rays = 2 * n_per * args.steps
+    value = rays / wall / 1e6
+    avg_launch_ms = float(np.mean(list(kernel_ms.values())))
+    achieved = BYTES_PER_RAY * n_per / (avg_launch_ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    out = {
+        "metric": "Mrays/s + Mpixel·spp/s on 1 MI355X; p50/p99 job latency, served vs plain",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "closest-hit microbenchmark: trefoil tube 512x32 segments (32768 FP32 triangles), "
+                               "2^24 coherent + 2^24 incoherent rays per step",
+                   "rays_per_step": 2 * n_per, "parallelism": "single GPU"},
+        "per_set_mrays_s": {k: round(n_per / (v * 1e-3) / 1e6, 1) for k, v in kernel_ms.items()},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "kernel": "kClosest", "avg_launch_ms": round(avg_launch_ms, 4)},
+    }
+    hits = d_hits["incoherent"][:4096].cpu().numpy().view(np.uint32)
+    if not np.any(hits[:, 1] != 0xFFFFFFFF):
+        raise RuntimeError("benchmark produced no hits")
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(scene, [sets["coherent"], sets["incoherent"]])
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
