@@ -147,3 +147,96 @@ class GpuScene:
 def hits_t(hits: np.ndarray) -> np.ndarray:
     """The t column of a hits array as float32."""
     return hits[:, 0].copy().view(np.float32)
+
+
+class Server:
+    """The serving loop of a context (ab_server_start): submit jobs, poll
+    progressive results, wait, cancel. Jobs are (GpuScene, CameraSettings,
+    width, height, samples_per_axis, max_depth, roulette_start, seed)."""
+
+    QUEUED, RUNNING, DONE, CANCELLED, FAILED = 0, 1, 2, 3, 4
+
+    def __init__(self, ctx: Context, hbm_budget_bytes: int):
+        self.ctx = ctx
+        ctx.check(ctx.lib.ab_server_start(ctx.handle, hbm_budget_bytes))
+        self.shapes = {}
+
+    def stop(self):
+        if self.ctx is not None:
+            self.ctx.check(self.ctx.lib.ab_server_stop(self.ctx.handle))
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.stop()
+
+    @staticmethod
+    def job_desc(scene: GpuScene, camera: CameraSettings, width, height, samples_per_axis, max_depth=64,
+                 roulette_start=3, seed=0):
+        from .workloads import render_params
+        d = _abi.JobDesc()
+        d.scene = scene.handle
+        d.camera = camera.to_abi()
+        d.params = render_params(width, height, samples_per_axis, max_depth, roulette_start, seed)
+        return d
+
+    def submit(self, scene, camera, width, height, samples_per_axis, max_depth=64, roulette_start=3, seed=0):
+        d = self.job_desc(scene, camera, width, height, samples_per_axis, max_depth, roulette_start, seed)
+        jid = ctypes.c_uint64()
+        self.ctx.check(self.ctx.lib.ab_job_submit(self.ctx.handle, ctypes.byref(d), ctypes.byref(jid)))
+        self.shapes[jid.value] = (height, width)
+        return jid.value
+
+    def _status(self, st):
+        return {"state": st.state, "samples_done": st.samples_done, "samples_total": st.samples_total,
+                "submit_ms": st.submit_ms, "start_ms": st.start_ms, "finish_ms": st.finish_ms}
+
+    def poll(self, jid, image=True):
+        """(status code, status dict, image or None)."""
+        h, w = self.shapes[jid]
+        out = np.zeros((h, w, 3), np.float32) if image else None
+        st = _abi.JobStatus()
+        code = self.ctx.lib.ab_job_poll(self.ctx.handle, jid, ctypes.byref(st),
+                                        out.ctypes.data if image else None)
+        if code not in (_abi.AB_OK, _abi.AB_PENDING, _abi.AB_CANCELLED):
+            self.ctx.check(code)
+        return code, self._status(st), out
+
+    def wait(self, jid):
+        h, w = self.shapes[jid]
+        out = np.zeros((h, w, 3), np.float32)
+        st = _abi.JobStatus()
+        code = self.ctx.lib.ab_job_wait(self.ctx.handle, jid, ctypes.byref(st), out.ctypes.data)
+        if code not in (_abi.AB_OK, _abi.AB_CANCELLED):
+            self.ctx.check(code)
+        return code, self._status(st), out
+
+    def cancel(self, jid):
+        self.ctx.check(self.ctx.lib.ab_job_cancel(self.ctx.handle, jid))
+
+    def release(self, jid):
+        self.ctx.check(self.ctx.lib.ab_job_release(self.ctx.handle, jid))
+        self.shapes.pop(jid, None)
+
+    def stats(self):
+        s = _abi.ServerStats()
+        self.ctx.check(self.ctx.lib.ab_server_stats_get(self.ctx.handle, ctypes.byref(s)))
+        return {"jobs_done": s.jobs_done, "jobs_rejected": s.jobs_rejected, "rays": s.rays, "busy_ms": s.busy_ms,
+                "p50_latency_ms": s.p50_latency_ms, "p99_latency_ms": s.p99_latency_ms}
+
+
+def render_batch(ctx: Context, jobs):
+    """jobs: list of (GpuScene, CameraSettings, width, height, spa, max_depth, rr, seed).
+    Returns (images, stats dict) from one ab_render_batch call."""
+    descs = (_abi.JobDesc * len(jobs))()
+    outs = []
+    for i, j in enumerate(jobs):
+        descs[i] = Server.job_desc(*j)
+        outs.append(np.zeros((j[3], j[2], 3), np.float32))
+    ptrs = (ctypes.c_void_p * len(jobs))(*[o.ctypes.data for o in outs])
+    st = _abi.RenderStats()
+    ctx.check(ctx.lib.ab_render_batch(ctx.handle, descs, len(jobs), ptrs, ctypes.byref(st)))
+    return outs, {"camera_rays": st.camera_rays, "extension_rays": st.extension_rays, "shadow_rays": st.shadow_rays,
+                  "paths": st.paths, "gpu_ms": st.gpu_ms}

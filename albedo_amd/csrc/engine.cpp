@@ -242,7 +242,6 @@ ab_status Engine::renderBatch(const std::vector<JobSpec>& specs, const std::vect
         j.done = j.accum + size_t(j.width) * j.height * 4;
         off += align256(size_t(j.width) * j.height * 32 + 8);
     }
-    ps_.n_jobs = uint32_t(jobs_.size());
     work_total_ = total;
     HIP_TRY(hipMemcpyAsync(d_jobs_, jobs_.data(), sizeof(DevJob) * jobs_.size(), hipMemcpyHostToDevice, stream_),
             "hipMemcpy(jobs)");
@@ -252,7 +251,7 @@ ab_status Engine::renderBatch(const std::vector<JobSpec>& specs, const std::vect
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0), "hipEventCreate");
     HIP_TRY(hipEventCreate(&e1), "hipEventCreate");
-    HIP_TRY(launchReset(ps_, total, stream_), "reset launch");
+    HIP_TRY(launchReset(ps_, total, uint32_t(jobs_.size()), stream_), "reset launch");
     HIP_TRY(hipEventRecord(e0, stream_), "hipEventRecord");
     Counters c{};
     st = runUntilDone(shape, cfg.use_graph != 0, c);
@@ -290,6 +289,83 @@ ab_status Engine::renderBatch(const std::vector<JobSpec>& specs, const std::vect
         stats->gpu_ms = ms;
     }
     return st;
+}
+
+ab_status Engine::beginStream(const ab_launch_config& cfg, uint32_t max_jobs)
+{
+    const uint32_t capacity = std::max<uint32_t>(cfg.paths_in_flight, 1024);
+    ab_status st = reserve(capacity, max_jobs, nullptr);
+    if (st != AB_OK) return st;
+    jobs_.clear();
+    jobs_.reserve(max_jobs_);
+    work_total_ = 0;
+    shape_ = LaunchShape{cfg.block_size == 256 ? 128u : cfg.block_size, cfg.grid_blocks, 64u};
+    if (shape_.grid == 0) shape_.grid = std::max<uint32_t>(256, std::min<uint32_t>(capacity / shape_.block, 8192));
+    HIP_TRY(launchReset(ps_, 0, 0, stream_), "reset launch");
+    use_graph_ = cfg.use_graph != 0;
+    if (graph_) {
+        (void)hipGraphExecDestroy(graph_);
+        graph_ = nullptr;
+    }
+    if (use_graph_) {
+        hipGraph_t g = nullptr;
+        HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        const hipError_t le = launchBounce(ps_, shape_, stream_);
+        const hipError_t ce = hipStreamEndCapture(stream_, &g);
+        if (le != hipSuccess) return hipFail(le, "bounce capture");
+        HIP_TRY(ce, "hipStreamEndCapture");
+        const hipError_t ie = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIP_TRY(ie, "hipGraphInstantiate");
+    }
+    return AB_OK;
+}
+
+ab_status Engine::appendJob(const DevJob& prepared, unsigned long long* accum, unsigned long long* done,
+                            uint32_t* index)
+{
+    if (tableFull()) return fail(AB_REJECTED_BUDGET, "job table full");
+    if (prepared.scene.stack_depth > 64)
+        return fail(AB_INVALID_ARGUMENT, "scene BVH too deep for the wavefront engine (stack > 64)");
+    DevJob dj = prepared;
+    dj.work_begin = work_total_;
+    dj.accum = accum;
+    dj.done = done;
+    const uint32_t idx = uint32_t(jobs_.size());
+    jobs_.push_back(dj);
+    HIP_TRY(hipMemcpyAsync(d_jobs_ + idx, &jobs_[idx], sizeof(DevJob), hipMemcpyHostToDevice, stream_),
+            "hipMemcpy(job)");
+    work_total_ += dj.work_count;
+    HIP_TRY(launchAddWork(ps_, work_total_, uint32_t(jobs_.size()), stream_), "add-work launch");
+    if (index) *index = idx;
+    return AB_OK;
+}
+
+ab_status Engine::cancelJob(uint32_t index)
+{
+    if (index >= jobs_.size()) return fail(AB_NOT_FOUND, "no such engine job");
+    jobs_[index].cancelled = 1;
+    HIP_TRY(hipMemcpyAsync(&d_jobs_[index].cancelled, &jobs_[index].cancelled, sizeof(uint32_t),
+                           hipMemcpyHostToDevice, stream_),
+            "hipMemcpy(cancel)");
+    return AB_OK;
+}
+
+ab_status Engine::runBounces(uint32_t count)
+{
+    for (uint32_t k = 0; k < count; ++k) {
+        const ab_status st = bounceOnce(shape_);
+        if (st != AB_OK) return st;
+    }
+    return AB_OK;
+}
+
+ab_status Engine::resetTable()
+{
+    jobs_.clear();
+    work_total_ = 0;
+    HIP_TRY(launchReset(ps_, 0, 0, stream_), "reset launch");
+    return AB_OK;
 }
 
 Engine& contextEngine(ab_context* ctx)

@@ -35,11 +35,16 @@ public:
     ab_status renderBatch(const std::vector<JobSpec>& jobs, const std::vector<float*>& out, ab_render_stats* stats,
                           const ab_launch_config& cfg);
 
-    // Serving interface: jobs are appended while the engine runs.
-    ab_status beginStream(const ab_launch_config& cfg, uint32_t stack);
-    ab_status appendJob(const JobSpec& job, unsigned long long* accum, unsigned long long* done, uint32_t* index);
+    // Serving interface: jobs are appended while the engine runs. The job table
+    // holds `max_jobs` entries and is emptied by resetTable() once idle.
+    ab_status beginStream(const ab_launch_config& cfg, uint32_t max_jobs);
+    ab_status appendJob(const DevJob& prepared, unsigned long long* accum, unsigned long long* done,
+                        uint32_t* index);
     ab_status cancelJob(uint32_t index);
     ab_status runBounces(uint32_t count);
+    ab_status resetTable();
+    bool tableFull() const { return jobs_.size() >= max_jobs_; }
+    size_t tableSize() const { return jobs_.size(); }
     ab_status readCounters(Counters& out);
     bool idle(const Counters& c) const { return c.work_next == c.work_total && c.n_active == 0; }
     uint32_t capacity() const { return capacity_; }

@@ -1,14 +1,249 @@
-// Serving loop (filled in with the batched engine).
+// Serving loop: continuous batching of render jobs on one wavefront engine.
+//
+// No counterpart in the reference, which renders one RenderJob per call
+// (render/render_image.hpp:17-49). A host thread admits submitted jobs whenever
+// their HBM footprint fits the budget, appends them to the running engine's job
+// table (their paths fill free slots on the next bounce, next to the paths of
+// jobs already on the device), retires jobs as their last sample lands and
+// serves progressive snapshots of the per-pixel sums and counts.
+
 #include "context.hpp"
 #include "engine.hpp"
 
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <thread>
+#include <vector>
+
 namespace ab {
-struct Server {};
+
+namespace {
+
+enum JobState : uint32_t { kQueued = 0, kRunning = 1, kDone = 2, kCancelled = 3, kFailed = 4 };
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct ServerJob {
+    uint64_t id{0};
+    JobSpec spec;
+    DevJob prepared{};
+    uint32_t state{kQueued};
+    uint64_t total{0};
+    uint64_t done{0};
+    double submit_ms{0}, start_ms{0}, finish_ms{0};
+    size_t footprint{0};
+    void* dev{nullptr}; // accumulators (4 u64 per pixel) + done counter
+    uint32_t engine_index{0};
+    bool cancel_requested{false};
+    std::vector<float> result;
+};
+
+void meanFromSums(const std::vector<unsigned long long>& sums, size_t npx, float* out)
+{
+    for (size_t px = 0; px < npx; ++px) {
+        const double n = double(sums[4 * px + 3]);
+        for (int ch = 0; ch < 3; ++ch)
+            out[3 * px + ch] = n > 0 ? float(double(sums[4 * px + ch]) / 268435456.0 / n) : 0.0f;
+    }
+}
+
+} // namespace
+
+struct Server {
+    ab_context* ctx;
+    uint64_t budget;
+    Engine engine;
+    std::thread thread;
+    std::mutex mu;
+    std::condition_variable cv;
+    bool stop{false};
+    ab_status thread_status{AB_OK};
+    std::string thread_error;
+    std::map<uint64_t, std::shared_ptr<ServerJob>> jobs;
+    std::deque<uint64_t> pending;
+    std::vector<uint64_t> running;
+    std::vector<void*> retired; // buffers of cancelled jobs, freed once the engine drains
+    uint64_t used{0};
+    uint64_t next_id{1};
+    uint64_t jobs_done{0}, jobs_rejected{0};
+    unsigned long long rays_before{0}; // rays counted before the last table reset
+    unsigned long long rays_now{0};
+    double busy_ms{0};
+    std::vector<double> latencies;
+    hipStream_t poll_stream{nullptr};
+
+    Server(ab_context* c, uint64_t b) : ctx(c), budget(b), engine(c) {}
+
+    size_t footprintOf(const DevJob& j) const { return align256(size_t(j.width) * j.height * 32 + 8); }
+
+    void loop();
+    ab_status admitLocked();
+    void retireFinished();
+};
+
+ab_status Server::admitLocked()
+{
+    while (!pending.empty()) {
+        auto it = jobs.find(pending.front());
+        if (it == jobs.end() || it->second->state != kQueued) {
+            pending.pop_front();
+            continue;
+        }
+        ServerJob& j = *it->second;
+        if (used + j.footprint > budget || engine.tableFull()) break;
+        void* dev = nullptr;
+        hipError_t e = hipMalloc(&dev, j.footprint);
+        if (e != hipSuccess) break; // try again once something retires
+        e = hipMemsetAsync(dev, 0, j.footprint, engine.stream());
+        if (e != hipSuccess) {
+            (void)hipFree(dev);
+            return hipFail(e, "hipMemset(job)");
+        }
+        auto* acc = static_cast<unsigned long long*>(dev);
+        const size_t npx = size_t(j.prepared.width) * j.prepared.height;
+        uint32_t index = 0;
+        const ab_status st = engine.appendJob(j.prepared, acc, acc + 4 * npx, &index);
+        if (st != AB_OK) {
+            (void)hipFree(dev);
+            j.state = kFailed;
+            pending.pop_front();
+            cv.notify_all();
+            continue;
+        }
+        j.dev = dev;
+        j.engine_index = index;
+        j.state = kRunning;
+        j.start_ms = msSince(ctx);
+        used += j.footprint;
+        running.push_back(j.id);
+        pending.pop_front();
+    }
+    return AB_OK;
+}
+
+void Server::retireFinished()
+{
+    // Finished-sample counters of the running jobs, read outside the lock (only
+    // this thread frees job buffers).
+    std::vector<std::shared_ptr<ServerJob>> live;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        for (uint64_t id : running) live.push_back(jobs[id]);
+    }
+    for (auto& jp : live) {
+        ServerJob& j = *jp;
+        const size_t npx = size_t(j.prepared.width) * j.prepared.height;
+        unsigned long long d = 0;
+        if (hipMemcpy(&d, static_cast<unsigned long long*>(j.dev) + 4 * npx, 8, hipMemcpyDeviceToHost) != hipSuccess)
+            continue;
+        bool cancel = false;
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            j.done = d;
+            cancel = j.cancel_requested;
+        }
+        if (cancel) {
+            (void)engine.cancelJob(j.engine_index);
+            std::lock_guard<std::mutex> lock(mu);
+            j.state = kCancelled;
+            j.finish_ms = msSince(ctx);
+            retired.push_back(j.dev);
+            j.dev = nullptr;
+            used -= j.footprint;
+            running.erase(std::find(running.begin(), running.end(), j.id));
+            cv.notify_all();
+            continue;
+        }
+        if (d < j.total) continue;
+        std::vector<unsigned long long> sums(npx * 4);
+        const hipError_t e = hipMemcpy(sums.data(), j.dev, npx * 32, hipMemcpyDeviceToHost);
+        std::lock_guard<std::mutex> lock(mu);
+        if (e == hipSuccess) {
+            j.result.resize(npx * 3);
+            meanFromSums(sums, npx, j.result.data());
+            j.state = kDone;
+        } else {
+            j.state = kFailed;
+        }
+        j.finish_ms = msSince(ctx);
+        latencies.push_back(j.finish_ms - j.submit_ms);
+        ++jobs_done;
+        (void)hipFree(j.dev);
+        j.dev = nullptr;
+        used -= j.footprint;
+        running.erase(std::find(running.begin(), running.end(), j.id));
+        cv.notify_all();
+    }
+}
+
+void Server::loop()
+{
+    (void)hipSetDevice(ctx->device);
+    while (true) {
+        {
+            std::unique_lock<std::mutex> lock(mu);
+            if (stop) break;
+            const ab_status st = admitLocked();
+            if (st != AB_OK) {
+                thread_status = st;
+                break;
+            }
+            if (running.empty()) {
+                cv.wait_for(lock, std::chrono::milliseconds(2), [&] { return stop || !pending.empty(); });
+                continue;
+            }
+        }
+        const double t0 = msSince(ctx);
+        Counters c{};
+        if (engine.runBounces(8) != AB_OK || engine.readCounters(c) != AB_OK) {
+            std::lock_guard<std::mutex> lock(mu);
+            thread_status = AB_DEVICE_ERROR;
+            break;
+        }
+        busy_ms += msSince(ctx) - t0;
+        rays_now = c.extend_rays + c.shadow_rays;
+        retireFinished();
+        if (engine.idle(c)) {
+            std::lock_guard<std::mutex> lock(mu);
+            if (running.empty()) {
+                for (void* p : retired) (void)hipFree(p);
+                retired.clear();
+                if (engine.tableSize() > 0) {
+                    rays_before += rays_now;
+                    rays_now = 0;
+                    (void)engine.resetTable();
+                }
+            }
+        }
+    }
+    // Jobs left at shutdown are cancelled.
+    std::lock_guard<std::mutex> lock(mu);
+    (void)hipStreamSynchronize(engine.stream());
+    for (auto& kv : jobs) {
+        ServerJob& j = *kv.second;
+        if (j.state == kQueued || j.state == kRunning) j.state = kCancelled;
+        if (j.dev) (void)hipFree(j.dev);
+        j.dev = nullptr;
+    }
+    for (void* p : retired) (void)hipFree(p);
+    retired.clear();
+    running.clear();
+    cv.notify_all();
+}
+
 } // namespace ab
+
+using namespace ab;
 
 ab_context::~ab_context()
 {
-    delete server;
+    if (server) (void)ab_server_stop(this);
     delete engine;
     for (auto& kv : resident) {
         (void)hipFree(kv.second->arena);
@@ -16,3 +251,182 @@ ab_context::~ab_context()
     }
     if (stream) (void)hipStreamDestroy(stream);
 }
+
+extern "C" {
+
+ab_status ab_server_start(ab_context* ctx, uint64_t budget)
+{
+    if (!ctx) return fail(AB_INVALID_ARGUMENT, "ab_server_start: null context");
+    if (ctx->server) return fail(AB_INVALID_ARGUMENT, "ab_server_start: a server is already running");
+    if (budget == 0) return fail(AB_INVALID_ARGUMENT, "ab_server_start: HBM budget must be positive");
+    ab_launch_config cfg;
+    {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        cfg = ctx->config;
+    }
+    auto* s = new Server(ctx, budget);
+    ab_status st = s->engine.beginStream(cfg, 4096);
+    if (st != AB_OK) {
+        delete s;
+        return st;
+    }
+    ctx->server = s;
+    s->thread = std::thread([s] { s->loop(); });
+    return AB_OK;
+}
+
+ab_status ab_server_stop(ab_context* ctx)
+{
+    if (!ctx || !ctx->server) return fail(AB_INVALID_ARGUMENT, "ab_server_stop: no server running");
+    Server* s = ctx->server;
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        s->stop = true;
+    }
+    s->cv.notify_all();
+    if (s->thread.joinable()) s->thread.join();
+    ctx->server = nullptr;
+    delete s;
+    return AB_OK;
+}
+
+ab_status ab_job_submit(ab_context* ctx, const ab_job_desc* job, uint64_t* job_id)
+{
+    if (!ctx || !job || !job_id) return fail(AB_INVALID_ARGUMENT, "ab_job_submit: null argument");
+    Server* s = ctx->server;
+    if (!s) return fail(AB_INVALID_ARGUMENT, "ab_job_submit: no server running");
+    auto j = std::make_shared<ServerJob>();
+    j->spec = JobSpec{job->scene, job->camera, job->params};
+    if (job->scene && job->scene->ctx != ctx)
+        return fail(AB_INVALID_ARGUMENT, "ab_job_submit: scene belongs to another context");
+    const ab_status st = prepareJob(j->spec, j->prepared);
+    if (st != AB_OK) return st;
+    if (j->prepared.scene.stack_depth > 64)
+        return fail(AB_INVALID_ARGUMENT, "scene BVH too deep for the wavefront engine (stack > 64)");
+    j->total = j->prepared.work_count;
+    j->footprint = s->footprintOf(j->prepared);
+    std::lock_guard<std::mutex> lock(s->mu);
+    if (j->footprint > s->budget) {
+        ++s->jobs_rejected;
+        return fail(AB_REJECTED_BUDGET, "job needs " + std::to_string(j->footprint) + " bytes of HBM, budget is " +
+                                            std::to_string(s->budget));
+    }
+    j->id = s->next_id++;
+    j->submit_ms = msSince(ctx);
+    s->jobs[j->id] = j;
+    s->pending.push_back(j->id);
+    *job_id = j->id;
+    s->cv.notify_all();
+    return AB_OK;
+}
+
+static void fillStatus(const ServerJob& j, ab_job_status* st)
+{
+    if (!st) return;
+    st->state = j.state;
+    st->samples_done = j.state == kDone ? j.total : j.done;
+    st->samples_total = j.total;
+    st->submit_ms = j.submit_ms;
+    st->start_ms = j.start_ms;
+    st->finish_ms = j.finish_ms;
+}
+
+ab_status ab_job_poll(ab_context* ctx, uint64_t id, ab_job_status* status, float* rgb)
+{
+    if (!ctx || !ctx->server) return fail(AB_INVALID_ARGUMENT, "ab_job_poll: no server running");
+    Server* s = ctx->server;
+    std::lock_guard<std::mutex> lock(s->mu);
+    auto it = s->jobs.find(id);
+    if (it == s->jobs.end()) return fail(AB_NOT_FOUND, "ab_job_poll: unknown job " + std::to_string(id));
+    ServerJob& j = *it->second;
+    fillStatus(j, status);
+    const size_t npx = size_t(j.prepared.width) * j.prepared.height;
+    if (rgb) {
+        if (j.state == kDone) {
+            std::copy(j.result.begin(), j.result.end(), rgb);
+        } else if (j.state == kRunning && j.dev) {
+            // Progressive snapshot of sums and counts (the reference keeps both,
+            // image/pixel_buffer.hpp:54-57).
+            std::vector<unsigned long long> sums(npx * 4);
+            if (hipMemcpy(sums.data(), j.dev, npx * 32, hipMemcpyDeviceToHost) != hipSuccess)
+                return fail(AB_DEVICE_ERROR, "ab_job_poll: snapshot copy failed");
+            meanFromSums(sums, npx, rgb);
+        } else {
+            std::fill(rgb, rgb + npx * 3, 0.0f);
+        }
+    }
+    if (j.state == kCancelled) return AB_CANCELLED;
+    if (j.state == kFailed) return fail(AB_DEVICE_ERROR, "job failed");
+    return j.state == kDone ? AB_OK : AB_PENDING;
+}
+
+ab_status ab_job_wait(ab_context* ctx, uint64_t id, ab_job_status* status, float* rgb)
+{
+    if (!ctx || !ctx->server) return fail(AB_INVALID_ARGUMENT, "ab_job_wait: no server running");
+    Server* s = ctx->server;
+    {
+        std::unique_lock<std::mutex> lock(s->mu);
+        auto it = s->jobs.find(id);
+        if (it == s->jobs.end()) return fail(AB_NOT_FOUND, "ab_job_wait: unknown job " + std::to_string(id));
+        auto jp = it->second;
+        s->cv.wait(lock, [&] {
+            return jp->state == kDone || jp->state == kCancelled || jp->state == kFailed || s->stop ||
+                   s->thread_status != AB_OK;
+        });
+        if (s->thread_status != AB_OK) return fail(s->thread_status, "serving loop stopped on a device error");
+    }
+    return ab_job_poll(ctx, id, status, rgb);
+}
+
+ab_status ab_job_cancel(ab_context* ctx, uint64_t id)
+{
+    if (!ctx || !ctx->server) return fail(AB_INVALID_ARGUMENT, "ab_job_cancel: no server running");
+    Server* s = ctx->server;
+    std::lock_guard<std::mutex> lock(s->mu);
+    auto it = s->jobs.find(id);
+    if (it == s->jobs.end()) return fail(AB_NOT_FOUND, "ab_job_cancel: unknown job " + std::to_string(id));
+    ServerJob& j = *it->second;
+    if (j.state == kQueued) {
+        j.state = kCancelled;
+        s->cv.notify_all();
+    } else if (j.state == kRunning) {
+        j.cancel_requested = true;
+    }
+    return AB_OK;
+}
+
+ab_status ab_job_release(ab_context* ctx, uint64_t id)
+{
+    if (!ctx || !ctx->server) return fail(AB_INVALID_ARGUMENT, "ab_job_release: no server running");
+    Server* s = ctx->server;
+    std::lock_guard<std::mutex> lock(s->mu);
+    auto it = s->jobs.find(id);
+    if (it == s->jobs.end()) return fail(AB_NOT_FOUND, "ab_job_release: unknown job " + std::to_string(id));
+    if (it->second->state == kQueued || it->second->state == kRunning)
+        return fail(AB_INVALID_ARGUMENT, "ab_job_release: job still active; cancel or wait first");
+    s->jobs.erase(it);
+    return AB_OK;
+}
+
+ab_status ab_server_stats_get(ab_context* ctx, ab_server_stats* out)
+{
+    if (!ctx || !ctx->server || !out) return fail(AB_INVALID_ARGUMENT, "ab_server_stats_get: no server running");
+    Server* s = ctx->server;
+    std::lock_guard<std::mutex> lock(s->mu);
+    out->jobs_done = s->jobs_done;
+    out->jobs_rejected = s->jobs_rejected;
+    out->rays = s->rays_before + s->rays_now;
+    out->busy_ms = s->busy_ms;
+    std::vector<double> l = s->latencies;
+    std::sort(l.begin(), l.end());
+    auto q = [&](double p) {
+        if (l.empty()) return 0.0;
+        const size_t k = std::min(l.size() - 1, size_t(std::ceil(p * double(l.size()))) - (p > 0 ? 1 : 0));
+        return l[k];
+    };
+    out->p50_latency_ms = q(0.5);
+    out->p99_latency_ms = q(0.99);
+    return AB_OK;
+}
+
+} // extern "C"

@@ -68,7 +68,7 @@ __global__ void kGenerate(PathState ps)
     for (uint32_t i = gridStart(); i < n_new; i += gridStride()) {
         const uint32_t slot = ps.freeq[free_top - 1 - i];
         const unsigned long long w = w0 + i;
-        const uint32_t j = findJob(ps.jobs, ps.n_jobs, w);
+        const uint32_t j = findJob(ps.jobs, c->n_jobs, w);
         const DevJob& job = ps.jobs[j];
         const unsigned long long local = w - job.work_begin;
         const unsigned long long npx = uint64_t(job.width) * job.height;
@@ -538,7 +538,7 @@ __global__ void kPost(PathState ps)
     c->n_new = uint32_t(left < c->n_free ? left : c->n_free);
 }
 
-__global__ void kReset(PathState ps, unsigned long long total)
+__global__ void kReset(PathState ps, unsigned long long total, uint32_t n_jobs)
 {
     const uint32_t i0 = gridStart();
     for (uint32_t i = i0; i < ps.capacity; i += gridStride()) ps.freeq[i] = i;
@@ -547,14 +547,16 @@ __global__ void kReset(PathState ps, unsigned long long total)
         *c = Counters{};
         c->n_free = ps.capacity;
         c->work_total = total;
+        c->n_jobs = n_jobs;
         c->n_new = uint32_t(total < ps.capacity ? total : ps.capacity);
     }
 }
 
-__global__ void kAddWork(PathState ps, unsigned long long total)
+__global__ void kAddWork(PathState ps, unsigned long long total, uint32_t n_jobs)
 {
     Counters* c = ps.ctr;
     c->work_total = total;
+    c->n_jobs = n_jobs;
     const unsigned long long left = total - c->work_next;
     c->n_new = uint32_t(left < c->n_free ? left : c->n_free);
 }
@@ -597,15 +599,15 @@ hipError_t launchBounce(const PathState& ps, const LaunchShape& sh, hipStream_t 
     }
 }
 
-hipError_t launchReset(const PathState& ps, unsigned long long total, hipStream_t s)
+hipError_t launchReset(const PathState& ps, unsigned long long total, uint32_t n_jobs, hipStream_t s)
 {
-    hipLaunchKernelGGL(kReset, dim3(1024), dim3(256), 0, s, ps, total);
+    hipLaunchKernelGGL(kReset, dim3(1024), dim3(256), 0, s, ps, total, n_jobs);
     return hipGetLastError();
 }
 
-hipError_t launchAddWork(const PathState& ps, unsigned long long total, hipStream_t s)
+hipError_t launchAddWork(const PathState& ps, unsigned long long total, uint32_t n_jobs, hipStream_t s)
 {
-    hipLaunchKernelGGL(kAddWork, dim3(1), dim3(1), 0, s, ps, total);
+    hipLaunchKernelGGL(kAddWork, dim3(1), dim3(1), 0, s, ps, total, n_jobs);
     return hipGetLastError();
 }
 

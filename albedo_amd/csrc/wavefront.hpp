@@ -36,7 +36,8 @@ struct alignas(16) Counters {
     uint32_t bin_count[8];
     uint32_t bin_fill[8];
     uint32_t bin_off[8];
-    uint32_t pad[2];
+    uint32_t n_jobs;       // entries of the job table in use
+    uint32_t pad;
     unsigned long long work_next;
     unsigned long long work_total;
     unsigned long long camera_rays;
@@ -69,8 +70,8 @@ struct PathState {
     uint32_t* freeq;
     Counters* ctr;
     const DevJob* jobs;
-    uint32_t n_jobs;
     uint32_t capacity;
+    uint32_t pad;
 };
 
 struct LaunchShape {
@@ -83,8 +84,8 @@ struct LaunchShape {
 // bookkeeping kernels between them. No host synchronisation; capturable.
 hipError_t launchBounce(const PathState& ps, const LaunchShape& shape, hipStream_t s);
 // Initialises counters and the free list for a fresh engine.
-hipError_t launchReset(const PathState& ps, unsigned long long work_total, hipStream_t s);
+hipError_t launchReset(const PathState& ps, unsigned long long work_total, uint32_t n_jobs, hipStream_t s);
 // Adds newly admitted work to the total (serving).
-hipError_t launchAddWork(const PathState& ps, unsigned long long work_total, hipStream_t s);
+hipError_t launchAddWork(const PathState& ps, unsigned long long work_total, uint32_t n_jobs, hipStream_t s);
 
 } // namespace ab

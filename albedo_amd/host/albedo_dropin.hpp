@@ -1,0 +1,649 @@
+// C++ drop-in for the reference renderer's scene and render API, backed by the
+// MI355X library through its C ABI (include/albedo_gpu.h).
+//
+// Mirrors, by name and meaning:
+//   Transform                      core/transform.hpp:38-93
+//   Sphere, TriangleMesh           geometry/sphere.hpp, geometry/triangle_mesh.hpp
+//   MeshGeometry, MeshShading      geometry/mesh_geometry.hpp:21-66
+//   MeshInstance                   geometry/mesh_instance.hpp:18-66
+//   Constant/Checker/PerlinTexture texture/*.hpp
+//   Diffuse/Conductor/Dielectric/MixMaterial, PerlinPattern  material/*.hpp
+//   PointLight, SkyLight           light/point_light.hpp, light/sky_light.hpp
+//   Scene                          scene/scene.hpp:31-98 (add, addAreaLight, addLight, setSky)
+//   CameraSettings, PerspectiveCamera  camera/perspective_camera.hpp:16-53
+//   PathEstimator                  estimator/path_estimator.hpp:22-38
+//   RenderJob, renderImage         render/render_image.hpp:17-49
+//   PixelBuffer                    image/pixel_buffer.hpp
+// Error behaviour follows the reference: std::invalid_argument for invalid
+// arguments (AB_INVALID_ARGUMENT), std::bad_alloc for AB_OUT_OF_MEMORY, and
+// std::runtime_error for device failures.
+#pragma once
+
+#include "albedo_gpu.h"
+
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <optional>
+#include <stdexcept>
+#include <string>
+#include <variant>
+#include <vector>
+
+namespace albedo {
+
+using Real = double;
+
+struct Vector3r {
+    Real x{}, y{}, z{};
+};
+struct Point3r {
+    Real x{}, y{}, z{};
+};
+struct Point2r {
+    Real x{}, y{};
+};
+struct Point2i {
+    int x{}, y{};
+};
+struct Rgb {
+    Real r{}, g{}, b{};
+};
+
+namespace detail {
+
+inline void check(ab_status st)
+{
+    if (st == AB_OK) return;
+    const char* m = ab_last_error(nullptr);
+    const std::string msg = m ? m : "";
+    if (st == AB_INVALID_ARGUMENT || st == AB_PARSE_ERROR) throw std::invalid_argument(msg);
+    if (st == AB_OUT_OF_MEMORY) throw std::bad_alloc();
+    throw std::runtime_error(msg);
+}
+
+// The process-wide library context on HIP device 0, created on first use.
+inline ab_context* context()
+{
+    static std::once_flag once;
+    static ab_context* ctx = nullptr;
+    std::call_once(once, [] { check(ab_context_create(0, &ctx)); });
+    return ctx;
+}
+
+using Mat3 = std::array<std::array<Real, 3>, 3>;
+
+inline Mat3 identity() { return {{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}}; }
+
+inline Mat3 mul(const Mat3& a, const Mat3& b)
+{
+    Mat3 r{};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            Real s = 0;
+            for (int k = 0; k < 3; ++k) s += a[i][k] * b[k][j];
+            r[i][j] = s;
+        }
+    return r;
+}
+
+inline std::array<Real, 3> apply(const Mat3& m, const std::array<Real, 3>& v)
+{
+    return {m[0][0] * v[0] + m[0][1] * v[1] + m[0][2] * v[2], m[1][0] * v[0] + m[1][1] * v[1] + m[1][2] * v[2],
+            m[2][0] * v[0] + m[2][1] * v[1] + m[2][2] * v[2]};
+}
+
+} // namespace detail
+
+// Affine map with its exact inverse (reference core/transform.cpp:93-167).
+class Transform {
+public:
+    Transform() = default;
+
+    static Transform translation(Vector3r v)
+    {
+        if (!std::isfinite(v.x) || !std::isfinite(v.y) || !std::isfinite(v.z))
+            throw std::invalid_argument("transform translation component is not finite");
+        Transform t;
+        t.off_ = {v.x, v.y, v.z};
+        t.ioff_ = {-v.x, -v.y, -v.z};
+        return t;
+    }
+
+    static Transform scaling(Vector3r f)
+    {
+        Transform t;
+        const Real s[3] = {f.x, f.y, f.z};
+        for (int k = 0; k < 3; ++k) {
+            if (!std::isfinite(s[k]) || s[k] == 0 || !std::isfinite(1 / s[k]))
+                throw std::invalid_argument("transform scale factor must be finite and non-zero with a finite reciprocal");
+            t.lin_[k][k] = s[k];
+            t.ilin_[k][k] = 1 / s[k];
+        }
+        return t;
+    }
+
+    static Transform rotation(Vector3r axis, Real degrees)
+    {
+        const Real l2 = axis.x * axis.x + axis.y * axis.y + axis.z * axis.z;
+        if (!std::isfinite(degrees)) throw std::invalid_argument("transform rotation angle is not finite");
+        if (!std::isnormal(l2)) throw std::invalid_argument("transform rotation axis has no usable direction");
+        const Real l = std::sqrt(l2), x = axis.x / l, y = axis.y / l, z = axis.z / l;
+        const Real r = degrees * (3.14159265358979323846 / 180);
+        const Real c = std::cos(r), s = std::sin(r), t = 1 - c;
+        Transform out;
+        out.lin_ = {{{t * x * x + c, t * x * y - s * z, t * x * z + s * y},
+                     {t * x * y + s * z, t * y * y + c, t * y * z - s * x},
+                     {t * x * z - s * y, t * y * z + s * x, t * z * z + c}}};
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) out.ilin_[i][j] = out.lin_[j][i];
+        return out;
+    }
+
+    // `*this` applied after `first`.
+    Transform operator*(const Transform& first) const
+    {
+        Transform r;
+        r.lin_ = detail::mul(lin_, first.lin_);
+        const auto o = detail::apply(lin_, first.off_);
+        r.off_ = {o[0] + off_[0], o[1] + off_[1], o[2] + off_[2]};
+        r.ilin_ = detail::mul(first.ilin_, ilin_);
+        const auto io = detail::apply(first.ilin_, ioff_);
+        r.ioff_ = {io[0] + first.ioff_[0], io[1] + first.ioff_[1], io[2] + first.ioff_[2]};
+        for (int i = 0; i < 3; ++i) {
+            for (int j = 0; j < 3; ++j)
+                if (!std::isfinite(r.lin_[i][j]) || !std::isfinite(r.ilin_[i][j]))
+                    throw std::invalid_argument("composed transform has a non-finite entry");
+            if (!std::isfinite(r.off_[i]) || !std::isfinite(r.ioff_[i]))
+                throw std::invalid_argument("composed transform has a non-finite entry");
+        }
+        return r;
+    }
+
+    Transform inverse() const
+    {
+        Transform r;
+        r.lin_ = ilin_;
+        r.off_ = ioff_;
+        r.ilin_ = lin_;
+        r.ioff_ = off_;
+        return r;
+    }
+
+    Point3r apply(Point3r p) const
+    {
+        const auto v = detail::apply(lin_, {p.x, p.y, p.z});
+        return {v[0] + off_[0], v[1] + off_[1], v[2] + off_[2]};
+    }
+
+    void rows(double* fwd, double* inv) const
+    {
+        for (int r = 0; r < 3; ++r) {
+            for (int k = 0; k < 3; ++k) {
+                fwd[4 * r + k] = lin_[r][k];
+                inv[4 * r + k] = ilin_[r][k];
+            }
+            fwd[4 * r + 3] = off_[r];
+            inv[4 * r + 3] = ioff_[r];
+        }
+    }
+
+private:
+    detail::Mat3 lin_{detail::identity()}, ilin_{detail::identity()};
+    std::array<Real, 3> off_{}, ioff_{};
+};
+
+class Sphere {
+public:
+    Sphere(Point3r center, Real radius) : c_(center), r_(radius)
+    {
+        if (!(radius > 0) || !std::isfinite(radius) || !std::isnormal(radius * radius))
+            throw std::invalid_argument("sphere radius must be positive and its square a normal number");
+        if (!std::isfinite(center.x) || !std::isfinite(center.y) || !std::isfinite(center.z))
+            throw std::invalid_argument("sphere center must be finite");
+    }
+    Point3r center() const { return c_; }
+    Real radius() const { return r_; }
+
+private:
+    Point3r c_;
+    Real r_;
+};
+
+// Positions and triangle faces, with optional per-corner texture coordinates.
+class TriangleMesh {
+public:
+    using Face = std::array<std::size_t, 3>;
+    TriangleMesh(std::vector<Point3r> positions, std::vector<Face> faces, std::vector<Point2r> uv = {},
+                 std::vector<Face> uv_faces = {})
+        : positions_(std::move(positions)), faces_(std::move(faces)), uv_(std::move(uv)), uv_faces_(std::move(uv_faces))
+    {
+        for (const Face& f : faces_)
+            for (std::size_t i : f)
+                if (i >= positions_.size()) throw std::out_of_range("mesh face uses a position the mesh does not have");
+        if (!uv_faces_.empty() && uv_faces_.size() != faces_.size())
+            throw std::invalid_argument("mesh has a different number of texture faces and faces");
+        for (const Face& f : uv_faces_)
+            for (std::size_t i : f)
+                if (i >= uv_.size()) throw std::out_of_range("mesh texture face uses a missing coordinate");
+    }
+    const std::vector<Point3r>& positions() const { return positions_; }
+    const std::vector<Face>& faces() const { return faces_; }
+    const std::vector<Point2r>& textureCoordinates() const { return uv_; }
+    const std::vector<Face>& textureFaces() const { return uv_faces_; }
+
+private:
+    std::vector<Point3r> positions_;
+    std::vector<Face> faces_;
+    std::vector<Point2r> uv_;
+    std::vector<Face> uv_faces_;
+};
+
+enum class MeshShading : std::uint8_t { Flat, Smooth };
+
+class MeshGeometry {
+public:
+    explicit MeshGeometry(const TriangleMesh& mesh, MeshShading shading = MeshShading::Flat)
+        : shading_(shading)
+    {
+        pos_.reserve(mesh.positions().size() * 3);
+        for (const Point3r& p : mesh.positions()) {
+            pos_.push_back(float(p.x));
+            pos_.push_back(float(p.y));
+            pos_.push_back(float(p.z));
+        }
+        for (const auto& f : mesh.faces())
+            for (std::size_t i : f) idx_.push_back(uint32_t(i));
+        if (!mesh.textureFaces().empty()) {
+            for (const auto& f : mesh.textureFaces())
+                for (std::size_t i : f) {
+                    uv_.push_back(float(mesh.textureCoordinates()[i].x));
+                    uv_.push_back(float(mesh.textureCoordinates()[i].y));
+                }
+        }
+    }
+    MeshShading shading() const { return shading_; }
+    const std::vector<float>& positions() const { return pos_; }
+    const std::vector<uint32_t>& indices() const { return idx_; }
+    const std::vector<float>& uv() const { return uv_; }
+
+private:
+    MeshShading shading_;
+    std::vector<float> pos_;
+    std::vector<uint32_t> idx_;
+    std::vector<float> uv_;
+};
+
+class MeshInstance {
+public:
+    MeshInstance(std::shared_ptr<const MeshGeometry> mesh, const Transform& object_to_world)
+        : mesh_(std::move(mesh)), to_world_(object_to_world)
+    {
+        if (!mesh_) throw std::invalid_argument("mesh instance needs a mesh");
+    }
+    const MeshGeometry& mesh() const { return *mesh_; }
+    const std::shared_ptr<const MeshGeometry>& meshPtr() const { return mesh_; }
+    const Transform& transform() const { return to_world_; }
+
+private:
+    std::shared_ptr<const MeshGeometry> mesh_;
+    Transform to_world_;
+};
+
+using Shape = std::variant<Sphere, MeshInstance>;
+
+struct ConstantTexture {
+    Rgb color{};
+};
+struct CheckerTexture {
+    Rgb even{};
+    Rgb odd{};
+    Real frequency{1};
+};
+struct PerlinTexture {
+    Rgb low{};
+    Rgb high{1, 1, 1};
+    Real scale{1};
+    Real veins{0};
+};
+using Texture = std::variant<ConstantTexture, CheckerTexture, PerlinTexture>;
+
+struct DiffuseMaterial {
+    Texture albedo;
+};
+struct ConductorMaterial {
+    Rgb eta{1, 1, 1};
+    Rgb k{};
+    Real roughness{};
+};
+struct DielectricMaterial {
+    Real ior{1.5};
+    Real roughness{};
+};
+struct PerlinPattern {
+    Real scale{1};
+    Real veins{0};
+};
+using BasicMaterial = std::variant<DiffuseMaterial, ConductorMaterial, DielectricMaterial>;
+struct MixMaterial {
+    BasicMaterial first;
+    BasicMaterial second;
+    PerlinPattern pattern;
+    Real threshold{0.5};
+};
+using Material = std::variant<DiffuseMaterial, ConductorMaterial, DielectricMaterial, MixMaterial>;
+
+struct PointLight {
+    Point3r position{};
+    Rgb intensity{};
+};
+struct SkyLight {
+    Rgb nadir{};
+    Rgb zenith{};
+    static SkyLight uniform(Rgb c) { return {c, c}; }
+};
+
+// Objects and lights; flattens into an ab_scene_desc when rendered.
+class Scene {
+public:
+    void add(const Shape& shape, const Material& material) { objects_.push_back({shape, material, -1}); }
+
+    void addAreaLight(const Sphere& sphere, Rgb radiance, const Material& material = DiffuseMaterial{})
+    {
+        const int light = int(lights_.size());
+        Light l;
+        l.kind = AB_LIGHT_AREA;
+        l.object = uint32_t(objects_.size());
+        l.radiance = radiance;
+        lights_.push_back(l);
+        objects_.push_back({sphere, material, light});
+    }
+
+    std::size_t addLight(const PointLight& light)
+    {
+        Light l;
+        l.kind = AB_LIGHT_POINT;
+        l.point = light;
+        lights_.push_back(l);
+        return lights_.size() - 1;
+    }
+
+    void setSky(const SkyLight& sky)
+    {
+        Light l;
+        l.kind = AB_LIGHT_SKY;
+        l.sky = sky;
+        if (sky_) {
+            lights_[*sky_] = l;
+        } else {
+            sky_ = lights_.size();
+            lights_.push_back(l);
+        }
+    }
+
+    std::size_t shapeCount() const { return objects_.size(); }
+
+    // The resident device scene (built on first use, shared by content hash).
+    ab_scene* device() const
+    {
+        std::lock_guard<std::mutex> lock(mu_);
+        if (!device_) device_.reset(build(), [](ab_scene* s) { ab_scene_release(s); });
+        return device_.get();
+    }
+
+    Scene() = default;
+    Scene(const Scene& o) : objects_(o.objects_), lights_(o.lights_), sky_(o.sky_) {}
+
+private:
+    struct Object {
+        Shape shape;
+        Material material;
+        int light;
+    };
+    struct Light {
+        uint32_t kind{};
+        uint32_t object{};
+        Rgb radiance{};
+        PointLight point{};
+        SkyLight sky{};
+    };
+
+    static void putRgb(float* d, Rgb c)
+    {
+        d[0] = float(c.r);
+        d[1] = float(c.g);
+        d[2] = float(c.b);
+    }
+
+    static ab_texture_desc texture(const Texture& t)
+    {
+        ab_texture_desc d{};
+        if (const auto* c = std::get_if<ConstantTexture>(&t)) {
+            d.kind = AB_TEX_CONSTANT;
+            putRgb(d.a, c->color);
+        } else if (const auto* k = std::get_if<CheckerTexture>(&t)) {
+            d.kind = AB_TEX_CHECKER;
+            putRgb(d.a, k->even);
+            putRgb(d.b, k->odd);
+            d.frequency = float(k->frequency);
+        } else {
+            const auto& p = std::get<PerlinTexture>(t);
+            d.kind = AB_TEX_PERLIN;
+            putRgb(d.a, p.low);
+            putRgb(d.b, p.high);
+            d.frequency = float(p.scale);
+            d.veins = float(p.veins);
+        }
+        return d;
+    }
+
+    static ab_material_desc basic(const BasicMaterial& m)
+    {
+        ab_material_desc d{};
+        if (const auto* df = std::get_if<DiffuseMaterial>(&m)) {
+            d.kind = AB_MAT_DIFFUSE;
+            d.albedo = texture(df->albedo);
+        } else if (const auto* c = std::get_if<ConductorMaterial>(&m)) {
+            d.kind = AB_MAT_CONDUCTOR;
+            putRgb(d.eta, c->eta);
+            putRgb(d.k, c->k);
+            d.roughness = float(c->roughness);
+        } else {
+            const auto& e = std::get<DielectricMaterial>(m);
+            d.kind = AB_MAT_DIELECTRIC;
+            d.ior = float(e.ior);
+            d.roughness = float(e.roughness);
+        }
+        return d;
+    }
+
+    ab_scene* build() const
+    {
+        std::vector<ab_material_desc> mats;
+        std::vector<ab_mesh_desc> meshes;
+        std::vector<const MeshGeometry*> mesh_ptrs;
+        std::vector<ab_object_desc> objs;
+        for (const Object& o : objects_) {
+            ab_object_desc od{};
+            od.light = o.light;
+            od.material = uint32_t(mats.size());
+            if (const auto* mix = std::get_if<MixMaterial>(&o.material)) {
+                mats.push_back(basic(mix->first));
+                mats.push_back(basic(mix->second));
+                ab_material_desc m{};
+                m.kind = AB_MAT_MIX;
+                m.first = od.material;
+                m.second = od.material + 1;
+                m.scale = float(mix->pattern.scale);
+                m.veins = float(mix->pattern.veins);
+                m.threshold = float(mix->threshold);
+                od.material = uint32_t(mats.size());
+                mats.push_back(m);
+            } else {
+                mats.push_back(std::visit(
+                    [](const auto& b) -> ab_material_desc {
+                        using T = std::decay_t<decltype(b)>;
+                        if constexpr (std::is_same_v<T, MixMaterial>) return {};
+                        else return basic(BasicMaterial{b});
+                    },
+                    o.material));
+            }
+            if (const auto* s = std::get_if<Sphere>(&o.shape)) {
+                od.kind = AB_OBJ_SPHERE;
+                od.center[0] = s->center().x;
+                od.center[1] = s->center().y;
+                od.center[2] = s->center().z;
+                od.radius = s->radius();
+            } else {
+                const auto& inst = std::get<MeshInstance>(o.shape);
+                od.kind = AB_OBJ_INSTANCE;
+                const MeshGeometry* g = &inst.mesh();
+                std::size_t mi = 0;
+                while (mi < mesh_ptrs.size() && mesh_ptrs[mi] != g) ++mi;
+                if (mi == mesh_ptrs.size()) {
+                    mesh_ptrs.push_back(g);
+                    ab_mesh_desc md{};
+                    md.positions = g->positions().data();
+                    md.num_vertices = uint32_t(g->positions().size() / 3);
+                    md.indices = g->indices().data();
+                    md.num_faces = uint32_t(g->indices().size() / 3);
+                    md.uv = g->uv().empty() ? nullptr : g->uv().data();
+                    md.smooth = g->shading() == MeshShading::Smooth;
+                    meshes.push_back(md);
+                }
+                od.mesh = uint32_t(mi);
+                inst.transform().rows(od.to_world, od.to_object);
+            }
+            objs.push_back(od);
+        }
+        std::vector<ab_light_desc> lights;
+        for (const Light& l : lights_) {
+            ab_light_desc d{};
+            d.kind = l.kind;
+            d.object = l.object;
+            d.position[0] = l.point.position.x;
+            d.position[1] = l.point.position.y;
+            d.position[2] = l.point.position.z;
+            putRgb(d.intensity, l.point.intensity);
+            putRgb(d.radiance, l.radiance);
+            putRgb(d.nadir, l.sky.nadir);
+            putRgb(d.zenith, l.sky.zenith);
+            lights.push_back(d);
+        }
+        ab_scene_desc desc{meshes.data(), uint32_t(meshes.size()), objs.data(), uint32_t(objs.size()),
+                           mats.data(), uint32_t(mats.size()), lights.data(), uint32_t(lights.size())};
+        ab_scene* s = nullptr;
+        detail::check(ab_scene_create(detail::context(), &desc, &s));
+        return s;
+    }
+
+    std::vector<Object> objects_;
+    std::vector<Light> lights_;
+    std::optional<std::size_t> sky_;
+    mutable std::mutex mu_;
+    mutable std::shared_ptr<ab_scene> device_;
+};
+
+struct CameraSettings {
+    Point3r look_from{};
+    Point3r look_at{};
+    Vector3r up{0.0, 1.0, 0.0};
+    Real vertical_fov_degrees{60.0};
+    Real aspect_ratio{16.0 / 9.0};
+};
+
+class PerspectiveCamera {
+public:
+    explicit PerspectiveCamera(const CameraSettings& s) : s_(s)
+    {
+        if (std::isnan(s.vertical_fov_degrees) || s.vertical_fov_degrees <= 0 || s.vertical_fov_degrees >= 180)
+            throw std::invalid_argument("vertical field of view must be in (0, 180) degrees");
+        if (!(s.aspect_ratio > 0) || !std::isnormal(s.aspect_ratio))
+            throw std::invalid_argument("aspect ratio must be a positive normal number");
+    }
+    const CameraSettings& settings() const { return s_; }
+
+private:
+    CameraSettings s_;
+};
+
+struct PathEstimator {
+    int max_depth{64};
+    int roulette_start{3};
+};
+using RadianceEstimator = std::variant<PathEstimator>;
+
+// Per-pixel mean radiance (the reference keeps sums and counts; the device keeps
+// them as fixed-point sums and returns the means).
+class PixelBuffer {
+public:
+    PixelBuffer(int width, int height) : w_(width), h_(height)
+    {
+        if (width <= 0 || height <= 0 || int64_t(width) * height > (int64_t(1) << 28))
+            throw std::invalid_argument("invalid image size");
+        mean_.assign(std::size_t(width) * height * 3, 0.0f);
+    }
+    int width() const { return w_; }
+    int height() const { return h_; }
+    Rgb average(Point2i p) const
+    {
+        const std::size_t i = (std::size_t(p.y) * w_ + p.x) * 3;
+        return {mean_[i], mean_[i + 1], mean_[i + 2]};
+    }
+    float* data() { return mean_.data(); }
+    const float* data() const { return mean_.data(); }
+
+private:
+    int w_, h_;
+    std::vector<float> mean_;
+};
+
+struct RenderJob {
+    std::reference_wrapper<const PerspectiveCamera> camera;
+    std::reference_wrapper<const Scene> scene;
+    int samples_per_axis{1};
+    std::uint64_t seed{0};
+    RadianceEstimator estimator{PathEstimator{}};
+    int thread_count{1}; // accepted for source compatibility; the GPU ignores it
+};
+
+// Renders `job` into `image` (reference render/render_image.cpp:58-77).
+inline void renderImage(const RenderJob& job, PixelBuffer& image)
+{
+    if (job.samples_per_axis <= 0)
+        throw std::invalid_argument("samples per axis must be positive, got " + std::to_string(job.samples_per_axis));
+    if (job.thread_count <= 0)
+        throw std::invalid_argument("thread count must be positive, got " + std::to_string(job.thread_count));
+    const PathEstimator& est = std::get<PathEstimator>(job.estimator);
+    if (est.max_depth < 0 || est.roulette_start < 0)
+        throw std::invalid_argument("path limits must not be negative");
+    const CameraSettings& cs = job.camera.get().settings();
+    ab_camera cam{};
+    const Real from[3] = {cs.look_from.x, cs.look_from.y, cs.look_from.z};
+    const Real at[3] = {cs.look_at.x, cs.look_at.y, cs.look_at.z};
+    const Real up[3] = {cs.up.x, cs.up.y, cs.up.z};
+    for (int k = 0; k < 3; ++k) {
+        cam.look_from[k] = from[k];
+        cam.look_at[k] = at[k];
+        cam.up[k] = up[k];
+    }
+    cam.vfov_degrees = cs.vertical_fov_degrees;
+    cam.aspect = cs.aspect_ratio;
+    ab_render_params p{};
+    p.width = uint32_t(image.width());
+    p.height = uint32_t(image.height());
+    p.samples_per_axis = uint32_t(job.samples_per_axis);
+    p.max_depth = uint32_t(est.max_depth);
+    p.roulette_start = uint32_t(est.roulette_start);
+    p.seed = job.seed;
+    detail::check(ab_render(job.scene.get().device(), &cam, &p, image.data(), nullptr));
+}
+
+inline int availableThreadCount() noexcept { return 1; }
+
+} // namespace albedo

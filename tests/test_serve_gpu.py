@@ -1,0 +1,100 @@
+"""GPU tests of the serving loop (ab_server_*, ab_job_*): results equal the
+plain renderer's bit for bit, the HBM budget admits and rejects, cancellation,
+progressive snapshots, latency statistics."""
+import time
+
+import numpy as np
+import pytest
+
+from albedo_amd import AlbedoError, Server, _abi
+from albedo_amd import workloads as W
+from albedo_amd.scene import CameraSettings
+
+pytestmark = pytest.mark.gpu
+
+
+def cam(aspect, fov=38.0):
+    c = W.ROOM_CAMERA
+    return CameraSettings(c.look_from, c.look_at, (0, 1, 0), fov, aspect)
+
+
+@pytest.fixture(scope="module")
+def scenes(ctx):
+    return ctx.scene(W.room_scene()), ctx.scene(W.full_scene())
+
+
+def test_served_jobs_equal_plain_renders(ctx, scenes):
+    room, full = scenes
+    jobs = [(room, cam(40 / 24), 40, 24, 2, 8, 3, 1), (full, cam(64 / 36), 64, 36, 3, 16, 3, 2),
+            (room, cam(1.0, 50.0), 17, 17, 4, 4, 0, 9), (full, cam(2.0), 32, 16, 2, 8, 3, 4)]
+    with Server(ctx, 1 << 30) as srv:
+        ids = [srv.submit(*j) for j in jobs]
+        results = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    for (g, c, w, h, n, d, rr, seed), (code, status, img) in zip(jobs, results):
+        assert code == _abi.AB_OK and status["state"] == Server.DONE
+        assert status["samples_done"] == w * h * n * n
+        single = g.render(c, w, h, n, d, rr, seed)
+        assert np.array_equal(single.view(np.uint32), img.view(np.uint32))
+    assert st["jobs_done"] == len(jobs)
+    assert st["p99_latency_ms"] >= st["p50_latency_ms"] > 0
+    assert st["rays"] > 0
+
+
+def test_budget_rejects_and_serialises(ctx, scenes):
+    room, _ = scenes
+    one_job = 64 * 64 * 32 + 256  # footprint of a 64x64 job, rounded up
+    with Server(ctx, one_job) as srv:
+        with pytest.raises(AlbedoError, match="AB_REJECTED_BUDGET"):
+            srv.submit(room, cam(1.0), 128, 128, 1, 4)
+        a = srv.submit(room, cam(1.0), 64, 64, 4, 8)
+        b = srv.submit(room, cam(1.0), 64, 64, 4, 8, seed=1)
+        _, sa, _ = srv.wait(a)
+        _, sb, _ = srv.wait(b)
+        # Only one fits the budget: the second starts after the first finished.
+        assert sb["start_ms"] >= sa["finish_ms"]
+        assert srv.stats()["jobs_rejected"] == 1
+
+
+def test_cancel_and_progressive_snapshots(ctx, scenes):
+    room, full = scenes
+    with Server(ctx, 1 << 31) as srv:
+        big = srv.submit(full, cam(16 / 9), 640, 360, 16, 16)
+        small = srv.submit(room, cam(1.0), 32, 32, 2, 8)
+        seen = []
+        for _ in range(200):
+            code, st, img = srv.poll(big)
+            seen.append(st["samples_done"])
+            if st["samples_done"] > 0:
+                break
+            time.sleep(0.01)
+        srv.cancel(big)
+        code, st, _ = srv.wait(big)
+        assert code == _abi.AB_CANCELLED and st["state"] == Server.CANCELLED
+        code, st, img = srv.wait(small)
+        assert code == _abi.AB_OK
+        assert np.array_equal(img.view(np.uint32), room.render(cam(1.0), 32, 32, 2, 8).view(np.uint32))
+        assert seen == sorted(seen)
+        srv.release(big)
+        with pytest.raises(AlbedoError, match="AB_NOT_FOUND"):
+            srv.poll(big)
+
+
+def test_progressive_snapshot_converges_to_final(ctx, scenes):
+    room, _ = scenes
+    with Server(ctx, 1 << 30) as srv:
+        j = srv.submit(room, cam(16 / 9), 160, 90, 16, 8)
+        partial = None
+        for _ in range(500):
+            code, st, img = srv.poll(j)
+            if code == _abi.AB_PENDING and 0 < st["samples_done"] < st["samples_total"]:
+                partial = img
+            if code == _abi.AB_OK:
+                break
+            time.sleep(0.002)
+        code, st, final = srv.wait(j)
+        assert code == _abi.AB_OK
+        if partial is not None:
+            assert np.all(np.isfinite(partial))
+            rel = abs(float(partial.mean()) - float(final.mean())) / float(final.mean())
+            assert rel < 0.5
