@@ -8,6 +8,7 @@ Python mirror of the reference renderer's interface for tests and benchmarks:
 All compute goes through the HIP library; there is no CPU fallback.
 """
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -31,9 +32,13 @@ class Context:
         h = ctypes.c_void_p()
         check(self.lib.ab_context_create(device, ctypes.byref(h)))
         self.handle = h
+        self._scenes = weakref.WeakSet()
 
     def close(self):
         if self.handle:
+            # Destroying the context frees its scenes: their handles die with it.
+            for s in list(self._scenes):
+                s.handle = None
             self.lib.ab_context_destroy(self.handle)
             self.handle = None
 
@@ -74,11 +79,12 @@ class GpuScene:
         h = ctypes.c_void_p()
         ctx.check(ctx.lib.ab_scene_create(ctx.handle, self.desc.ptr(), ctypes.byref(h)))
         self.handle = h
+        ctx._scenes.add(self)
 
     def release(self):
-        if self.handle:
+        if self.handle and self.ctx.handle:
             self.ctx.lib.ab_scene_release(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:

@@ -21,6 +21,7 @@
 
 #include "albedo_gpu.h"
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdint>
@@ -181,6 +182,24 @@ public:
         return {v[0] + off_[0], v[1] + off_[1], v[2] + off_[2]};
     }
 
+    // Arvo's transformed box (reference core/transform.cpp:118-132, 212-225).
+    void applyBox(const Point3r& lo, const Point3r& hi, Point3r& out_lo, Point3r& out_hi) const
+    {
+        const Real l[3] = {lo.x, lo.y, lo.z}, h[3] = {hi.x, hi.y, hi.z};
+        Real mn[3], mx[3];
+        for (int r = 0; r < 3; ++r) {
+            for (int k = 0; k < 3; ++k) {
+                const Real a = lin_[r][k] * l[k], b = lin_[r][k] * h[k];
+                mn[r] = k == 0 ? std::min(a, b) : mn[r] + std::min(a, b);
+                mx[r] = k == 0 ? std::max(a, b) : mx[r] + std::max(a, b);
+            }
+            mn[r] += off_[r];
+            mx[r] += off_[r];
+        }
+        out_lo = {mn[0], mn[1], mn[2]};
+        out_hi = {mx[0], mx[1], mx[2]};
+    }
+
     void rows(double* fwd, double* inv) const
     {
         for (int r = 0; r < 3; ++r) {
@@ -251,6 +270,30 @@ public:
     explicit MeshGeometry(const TriangleMesh& mesh, MeshShading shading = MeshShading::Flat)
         : shading_(shading)
     {
+        // Box and used points of the triangles with computable area
+        // (reference triangle_mesh.cpp:76-90, mesh_geometry.cpp:30-45).
+        const auto& P = mesh.positions();
+        std::vector<bool> used(P.size(), false);
+        for (const auto& f : mesh.faces()) {
+            const Point3r a = P[f[0]], b = P[f[1]], c = P[f[2]];
+            const Real e1[3] = {b.x - a.x, b.y - a.y, b.z - a.z}, e2[3] = {c.x - a.x, c.y - a.y, c.z - a.z};
+            const Real n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                               e1[0] * e2[1] - e1[1] * e2[0]};
+            if (!std::isnormal(n[0] * n[0] + n[1] * n[1] + n[2] * n[2])) continue;
+            for (std::size_t i : f) used[i] = true;
+        }
+        bool any = false;
+        for (std::size_t i = 0; i < P.size(); ++i) {
+            if (!used[i]) continue;
+            used_.push_back(P[i]);
+            if (!any) {
+                lo_ = hi_ = P[i];
+                any = true;
+            }
+            lo_ = {std::min(lo_.x, P[i].x), std::min(lo_.y, P[i].y), std::min(lo_.z, P[i].z)};
+            hi_ = {std::max(hi_.x, P[i].x), std::max(hi_.y, P[i].y), std::max(hi_.z, P[i].z)};
+        }
+        if (!any) throw std::invalid_argument("mesh geometry needs at least one triangle with area");
         pos_.reserve(mesh.positions().size() * 3);
         for (const Point3r& p : mesh.positions()) {
             pos_.push_back(float(p.x));
@@ -268,12 +311,17 @@ public:
         }
     }
     MeshShading shading() const { return shading_; }
+    Point3r boundsMin() const { return lo_; }
+    Point3r boundsMax() const { return hi_; }
+    const std::vector<Point3r>& usedPoints() const { return used_; }
     const std::vector<float>& positions() const { return pos_; }
     const std::vector<uint32_t>& indices() const { return idx_; }
     const std::vector<float>& uv() const { return uv_; }
 
 private:
     MeshShading shading_;
+    Point3r lo_{}, hi_{};
+    std::vector<Point3r> used_;
     std::vector<float> pos_;
     std::vector<uint32_t> idx_;
     std::vector<float> uv_;

@@ -278,6 +278,44 @@ def full_scene(seed=7, knot=None):
     return s
 
 
+def sphere_variant_scene(seed, n_spheres, base="room", knot=None):
+    """A seeded variant of the sphere room (base="room") or the full-feature scene
+    (base="full") holding `n_spheres` extra random spheres, each made of one of the
+    five room materials (smooth glass, frosted glass, gold, copper, silver) drawn
+    uniformly."""
+    s = room_scene(seed) if base == "room" else full_scene(seed, knot)
+    m = room_materials()
+    rng = Pcg32(seed, 21)
+    for _ in range(n_spheres):
+        r = rng.uniform(0.08, 0.35)
+        x = rng.uniform(-2.8 + r, 2.8 - r)
+        z = rng.uniform(-5.7 + r, 3.0)
+        y = r if rng.real() < 0.6 else rng.uniform(r, 4.5)
+        kind = FIVE_KINDS[min(int(rng.real() * 5), 4)]
+        s.add(Sphere((x, y, z), r), m[kind])
+    return s
+
+
+RESOLUTIONS = ((256, 144), (640, 360), (960, 540), (1920, 1080))
+SPAS = (2, 4, 8, 16)  # samples per axis: 4, 16, 64, 256 spp
+
+
+def job_mix(n_jobs, seed=3, resolutions=RESOLUTIONS, spas=SPAS, max_spheres=200, knot=None):
+    """The batched-rendering job list: resolutions and spp drawn uniformly from the
+    given sets, scenes seeded variants of the room and the full scene holding
+    5..max_spheres random spheres. Returns [(scene, (w, h), spa, max_depth, seed)]."""
+    rng = Pcg32(seed, 31)
+    jobs = []
+    for i in range(n_jobs):
+        w, h = resolutions[min(int(rng.real() * len(resolutions)), len(resolutions) - 1)]
+        spa = spas[min(int(rng.real() * len(spas)), len(spas) - 1)]
+        base = "room" if rng.real() < 0.5 else "full"
+        n_sph = 5 + int(rng.real() * (max_spheres - 4))
+        scene = sphere_variant_scene(1000 + i, n_sph, base, knot)
+        jobs.append((scene, (w, h), spa, 8 if base == "room" else 16, i))
+    return jobs
+
+
 def render_params(width, height, samples_per_axis, max_depth, roulette_start=3, seed=0):
     p = _abi.RenderParams()
     p.width, p.height = width, height

@@ -54,12 +54,103 @@ def cpu_baseline(scene, rays_sets, budget_s=12.0):
                       % (stride, done, elapsed)}
 
 
+def path_tracing_leg(ctx, cpu=True):
+    """Sphere room, 320x180 at 16 spp, max depth 8, roulette from bounce 3."""
+    from albedo_amd import workloads as W
+    from albedo_amd.scene import CameraSettings
+    s = W.room_scene()
+    g = ctx.scene(s)
+    c = W.ROOM_CAMERA
+    cam = CameraSettings(c.look_from, c.look_at, (0, 1, 0), 38.0, 320 / 180)
+    g.render(cam, 320, 180, 4, 8, 3, 0)  # warm-up (graph capture, allocation)
+    t0 = time.perf_counter()
+    reps = 5
+    st = None
+    for r in range(reps):
+        _, st = g.render(cam, 320, 180, 4, 8, 3, r, stats=True)
+    wall = (time.perf_counter() - t0) / reps
+    rays = st["camera_rays"] + st["extension_rays"] + st["shadow_rays"]
+    out = {"workload": "sphere room 320x180, 16 spp, max depth 8, roulette from 3",
+           "mpixel_spp_s": round(320 * 180 * 16 / wall / 1e6, 2), "mrays_s": round(rays / wall / 1e6, 2),
+           "ms_per_image": round(wall * 1e3, 2), "rays_per_image": int(rays), "gpu_ms": round(st["gpu_ms"], 2)}
+    if cpu:
+        import oracle
+        o = oracle.OracleScene(s.to_desc())
+        t0 = time.perf_counter()
+        _, counts = o.render(cam.to_abi(), W.render_params(320, 180, 4, 8, 3, 0), sample_mode=0)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"mpixel_spp_s": round(320 * 180 * 16 / dt / 1e6, 3),
+                               "mrays_s": round(float(counts.sum()) / dt / 1e6, 3), "cores": os.cpu_count(),
+                               "kind": "port", "sample": "the full image, FP64 oracle estimator, reference draw order"}
+    return out
+
+
+def batch_leg(ctx, n_jobs=16):
+    """Mixed jobs (resolutions 256x144..960x540, 4..64 spp, room / full-scene
+    variants with 5..200 random spheres): one-at-a-time vs one batched launch."""
+    import albedo_amd
+    from albedo_amd import workloads as W
+    from albedo_amd.scene import CameraSettings
+    knot = W.trefoil_mesh()
+    mix = W.job_mix(n_jobs, resolutions=W.RESOLUTIONS[:3], spas=W.SPAS[:3], knot=knot)
+    c = W.ROOM_CAMERA
+    jobs = []
+    for scene, (w, h), spa, depth, seed in mix:
+        jobs.append((ctx.scene(scene), CameraSettings(c.look_from, c.look_at, (0, 1, 0), 38.0, w / h), w, h, spa,
+                     depth, 3, seed))
+    albedo_amd.render_batch(ctx, jobs[:2])  # warm-up
+    t0 = time.perf_counter()
+    rays_plain = 0
+    for j in jobs:
+        _, st = j[0].render(*j[1:], stats=True)
+        rays_plain += st["camera_rays"] + st["extension_rays"] + st["shadow_rays"]
+    plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, st = albedo_amd.render_batch(ctx, jobs)
+    batched = time.perf_counter() - t0
+    rays_b = st["camera_rays"] + st["extension_rays"] + st["shadow_rays"]
+    samples = sum(j[2] * j[3] * j[4] ** 2 for j in jobs)
+    return {"workload": "%d mixed jobs, 256x144..960x540, 4..64 spp, 5..200 random spheres" % n_jobs,
+            "plain_s": round(plain, 3), "batched_s": round(batched, 3),
+            "plain_mrays_s": round(rays_plain / plain / 1e6, 1), "batched_mrays_s": round(rays_b / batched / 1e6, 1),
+            "batched_mpixel_spp_s": round(samples / batched / 1e6, 1),
+            "speedup": round(plain / batched, 3)}
+
+
+def serve_leg(ctx, n_jobs=24, rate_hz=40.0):
+    """Poisson arrivals of mixed jobs into the serving loop; p50/p99 latency."""
+    import albedo_amd
+    from albedo_amd import workloads as W
+    from albedo_amd.scene import CameraSettings
+    knot = W.trefoil_mesh()
+    mix = W.job_mix(n_jobs, seed=5, resolutions=W.RESOLUTIONS[:3], spas=W.SPAS[:3], knot=knot)
+    c = W.ROOM_CAMERA
+    scenes = [ctx.scene(s) for s, *_ in mix]
+    rng = np.random.default_rng(9)
+    gaps = rng.exponential(1.0 / rate_hz, n_jobs)
+    t0 = time.perf_counter()
+    with albedo_amd.Server(ctx, 8 << 30) as srv:
+        ids = []
+        for (scene, (w, h), spa, depth, seed), g, gap in zip(mix, scenes, gaps):
+            time.sleep(gap)
+            ids.append(srv.submit(g, CameraSettings(c.look_from, c.look_at, (0, 1, 0), 38.0, w / h), w, h, spa,
+                                  depth, 3, seed))
+        for i in ids:
+            srv.wait(i)
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    return {"workload": "%d mixed jobs, Poisson arrivals at %.0f/s" % (n_jobs, rate_hz),
+            "p50_latency_ms": round(st["p50_latency_ms"], 2), "p99_latency_ms": round(st["p99_latency_ms"], 2),
+            "mrays_s": round(st["rays"] / wall / 1e6, 1), "wall_s": round(wall, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--trace-only", action="store_true", help="closest-hit benchmark only")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: this path has no multi-GPU form (each GPU would trace its own "
@@ -136,6 +227,12 @@ def main():
         raise RuntimeError("benchmark produced no hits")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, [sets["coherent"], sets["incoherent"]])
+    del d_rays, d_hits
+    torch.cuda.empty_cache()
+    if not args.trace_only:
+        out["path_tracing"] = path_tracing_leg(ctx, cpu=not args.no_cpu)
+        out["batched"] = batch_leg(ctx)
+        out["served"] = serve_leg(ctx)
     print(json.dumps(out))
 
 

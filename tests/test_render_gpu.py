@@ -48,8 +48,8 @@ def test_room_same_streams_match_fp64_estimator(room):
     print("same-stream relative RMSE %.5f, rays gpu %s oracle %s" % (err, st, counts))
     assert err < 0.02
     # Ray counts follow the same paths up to rare divergences.
-    assert abs(st["extension_rays"] - counts[1]) <= 0.01 * counts[1]
-    assert abs(st["shadow_rays"] - counts[2]) <= 0.01 * counts[2]
+    assert abs(int(st["extension_rays"]) - int(counts[1])) <= 0.01 * int(counts[1])
+    assert abs(int(st["shadow_rays"]) - int(counts[2])) <= 0.01 * int(counts[2])
 
 
 def test_room_reference_sampling_at_4096_spp(room):
