@@ -201,7 +201,9 @@ class Server:
 
     def poll(self, jid, image=True):
         """(status code, status dict, image or None)."""
-        h, w = self.shapes[jid]
+        if jid not in self.shapes:
+            image = False
+        h, w = self.shapes.get(jid, (0, 0))
         out = np.zeros((h, w, 3), np.float32) if image else None
         st = _abi.JobStatus()
         code = self.ctx.lib.ab_job_poll(self.ctx.handle, jid, ctypes.byref(st),

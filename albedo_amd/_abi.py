@@ -156,6 +156,13 @@ def lib():
     """The loaded library with prototypes set. Raises if it is not built."""
     global _lib
     if _lib is None:
+        # PyTorch ships its own HIP runtime under the same soname. Whichever is
+        # loaded first serves the whole process, and torch only initialises on its
+        # own copy, so load torch (when installed) before the library.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("libalbedo_gpu.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
         L = ctypes.CDLL(LIB_PATH)

@@ -23,6 +23,14 @@ bool finite3(const double* p) { return std::isfinite(p[0]) && std::isfinite(p[1]
 // 2 jitter draws + at most 7 per bounce.
 constexpr uint32_t kMaxDepthLimit = uint32_t((1024 - 2) / 7);
 
+// Blocks of the grid-stride queue kernels: the configured count, or by default
+// enough for one pass over a full queue up to 2048 blocks (8 per CU).
+uint32_t gridFor(uint32_t configured, uint32_t capacity, uint32_t block)
+{
+    if (configured) return std::min(configured, kMaxGrid);
+    return std::max<uint32_t>(64, std::min<uint32_t>((capacity + block - 1) / block, 2048));
+}
+
 #define HIP_TRY(expr, what)                                   \
     do {                                                      \
         const hipError_t e_ = (expr);                         \
@@ -135,6 +143,8 @@ ab_status Engine::reserve(uint32_t capacity, uint32_t max_jobs, hipStream_t)
                     {reinterpret_cast<void**>(&p.sh_c), C * 16},     {reinterpret_cast<void**>(&p.active), C * 4},
                     {reinterpret_cast<void**>(&p.binned), C * 4},    {reinterpret_cast<void**>(&p.shadowq), C * 4},
                     {reinterpret_cast<void**>(&p.termq), C * 4},     {reinterpret_cast<void**>(&p.freeq), C * 4},
+                    {reinterpret_cast<void**>(&p.block_count), size_t(kMaxGrid) * 8 * 4},
+                    {reinterpret_cast<void**>(&p.block_off), size_t(kMaxGrid) * 8 * 4},
                     {reinterpret_cast<void**>(&p.ctr), sizeof(Counters)},
                     {reinterpret_cast<void**>(&d_jobs_), sizeof(DevJob) * std::max<uint32_t>(max_jobs, 1)}};
     size_t total = 0;
@@ -246,7 +256,7 @@ ab_status Engine::renderBatch(const std::vector<JobSpec>& specs, const std::vect
     HIP_TRY(hipMemcpyAsync(d_jobs_, jobs_.data(), sizeof(DevJob) * jobs_.size(), hipMemcpyHostToDevice, stream_),
             "hipMemcpy(jobs)");
     LaunchShape shape{cfg.block_size, cfg.grid_blocks, stack <= 32 ? 32u : 64u};
-    if (shape.grid == 0) shape.grid = std::max<uint32_t>(256, std::min<uint32_t>(capacity / shape.block, 8192));
+    shape.grid = gridFor(shape.grid, capacity, shape.block);
     if (shape.block == 256 && shape.stack > 32) shape.block = 128;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0), "hipEventCreate");
@@ -300,7 +310,7 @@ ab_status Engine::beginStream(const ab_launch_config& cfg, uint32_t max_jobs)
     jobs_.reserve(max_jobs_);
     work_total_ = 0;
     shape_ = LaunchShape{cfg.block_size == 256 ? 128u : cfg.block_size, cfg.grid_blocks, 64u};
-    if (shape_.grid == 0) shape_.grid = std::max<uint32_t>(256, std::min<uint32_t>(capacity / shape_.block, 8192));
+    shape_.grid = gridFor(shape_.grid, capacity, shape_.block);
     HIP_TRY(launchReset(ps_, 0, 0, stream_), "reset launch");
     use_graph_ = cfg.use_graph != 0;
     if (graph_) {
@@ -424,7 +434,8 @@ ab_status ab_set_launch_config(ab_context* ctx, const ab_launch_config* c)
         return fail(AB_INVALID_ARGUMENT, "block_size must be 64, 128 or 256");
     if (c->paths_in_flight < 1024 || c->paths_in_flight > (1u << 26))
         return fail(AB_INVALID_ARGUMENT, "paths_in_flight must be in [1024, 2^26]");
-    if (c->grid_blocks > (1u << 20)) return fail(AB_INVALID_ARGUMENT, "grid_blocks too large");
+    if (c->grid_blocks > kMaxGrid)
+        return fail(AB_INVALID_ARGUMENT, "grid_blocks must be at most " + std::to_string(kMaxGrid));
     std::lock_guard<std::mutex> lock(ctx->mu);
     ctx->config = *c;
     return AB_OK;

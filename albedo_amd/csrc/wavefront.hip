@@ -20,19 +20,6 @@ namespace {
 constexpr double kFixedScale = 268435456.0; // 2^28
 constexpr float kMaxSampleRadiance = 1e5f;  // per-sample clamp that keeps 64-bit sums exact
 
-// Position of this lane in a wave-aggregated append to *counter (lanes with pred).
-__device__ __forceinline__ uint32_t waveAppend(uint32_t* counter, bool pred)
-{
-    const unsigned long long mask = __ballot(pred);
-    if (mask == 0) return 0;
-    const int lane = __lane_id();
-    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, uint32_t(__popcll(mask)));
-    base = __shfl(base, leader);
-    return base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
-}
-
 __device__ __forceinline__ uint32_t findJob(const DevJob* jobs, uint32_t n, unsigned long long w)
 {
     uint32_t lo = 0, hi = n;
@@ -107,7 +94,118 @@ __global__ void kPostGenerate(PathState ps)
     c->work_next += n;
     c->camera_rays += n;
     c->n_new = 0;
-    for (int b = 0; b < 8; ++b) c->bin_count[b] = 0;
+}
+
+// ------------------------------------------------------ queue construction
+//
+// Queues are built without contended atomics: a producer kernel records, per
+// queue position i, which output queues the item goes to (a bit mask) and
+// counts them per block; a one-block plan kernel scans the per-block counts into
+// per-block output offsets; a scatter kernel walks the same positions in the
+// same block order and writes each item at offset + rank. Every device-wide
+// counter is then touched once per bounce instead of once per wave, and queue
+// order no longer depends on timing.
+
+// Lanes' bits of `flags` counted per queue over a block's loop, one count per wave.
+template <int NQ>
+struct WaveCounts {
+    uint32_t c[NQ];
+    __device__ __forceinline__ WaveCounts()
+    {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) c[q] = 0;
+    }
+    __device__ __forceinline__ void add(uint32_t flags)
+    {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) c[q] += uint32_t(__popcll(__ballot((flags >> q) & 1u)));
+    }
+};
+
+// Writes the block's per-queue totals to block_count[blockIdx.x * 8 + q].
+template <int BLOCK, int NQ>
+__device__ __forceinline__ void storeBlockCounts(const WaveCounts<NQ>& wc, uint32_t* block_count)
+{
+    __shared__ uint32_t s[BLOCK / 64][NQ];
+    const uint32_t wave = threadIdx.x >> 6;
+    if (__lane_id() == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) s[wave][q] = wc.c[q];
+    }
+    __syncthreads();
+    if (threadIdx.x < uint32_t(NQ)) {
+        uint32_t sum = 0;
+        for (uint32_t w = 0; w < uint32_t(BLOCK / 64); ++w) sum += s[w][threadIdx.x];
+        block_count[blockIdx.x * 8 + threadIdx.x] = sum;
+    }
+}
+
+// Exclusive scan of block_count over `grid` blocks for each of NQ queues into
+// block_off. concat: queues share one output array, queue q after queues < q.
+// Totals go to totals[q]. One block of 1024 threads.
+template <int NQ>
+__device__ __forceinline__ void scanBlockCounts(const uint32_t* block_count, uint32_t* block_off, uint32_t grid,
+                                                bool concat, uint32_t* totals)
+{
+    __shared__ uint32_t part[1024];
+    __shared__ uint32_t base;
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (grid + 1023) / 1024;
+    const uint32_t b0 = t * per, b1 = min(grid, b0 + per);
+    if (t == 0) base = 0;
+    for (int q = 0; q < NQ; ++q) {
+        uint32_t local = 0;
+        for (uint32_t b = b0; b < b1; ++b) local += block_count[b * 8 + q];
+        part[t] = local;
+        __syncthreads();
+        if (t == 0) {
+            uint32_t run = 0;
+            for (uint32_t k = 0; k < 1024; ++k) {
+                const uint32_t v = part[k];
+                part[k] = run;
+                run += v;
+            }
+            totals[q] = run;
+        }
+        __syncthreads();
+        uint32_t run = (concat ? base : 0) + part[t];
+        for (uint32_t b = b0; b < b1; ++b) {
+            block_off[b * 8 + q] = run;
+            run += block_count[b * 8 + q];
+        }
+        __syncthreads();
+        if (t == 0 && concat) base += totals[q];
+        __syncthreads();
+    }
+}
+
+// Rank of this lane among the block's lanes with bit q of `flags`, for one
+// loop iteration; total[q] receives the iteration's count. All threads call it.
+template <int BLOCK, int NQ>
+__device__ __forceinline__ void blockRanks(uint32_t flags, uint32_t (&rank)[NQ], uint32_t (&total)[NQ])
+{
+    __shared__ uint32_t s[BLOCK / 64][NQ];
+    const uint32_t wave = threadIdx.x >> 6, lane = __lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const unsigned long long m = __ballot((flags >> q) & 1u);
+        rank[q] = uint32_t(__popcll(m & below));
+        if (lane == 0) s[wave][q] = uint32_t(__popcll(m));
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        uint32_t before = 0, all = 0;
+        for (uint32_t w = 0; w < uint32_t(BLOCK / 64); ++w) {
+            const uint32_t v = s[w][q];
+            before += w < wave ? v : 0;
+            all += v;
+        }
+        rank[q] += before;
+        total[q] = all;
+    }
+    __syncthreads();
 }
 
 // ------------------------------------------------------------------ extend
@@ -122,65 +220,65 @@ template <int BLOCK, int STACK>
 __global__ __launch_bounds__(BLOCK) void kExtend(PathState ps)
 {
     __shared__ int32_t stack[STACK * BLOCK];
-    Counters* c = ps.ctr;
-    const uint32_t n = c->n_active;
-    for (uint32_t i = gridStart(); i < n; i += gridStride()) {
-        const uint32_t slot = ps.active[i];
-        const float4 o = ps.org[slot], d = ps.dir[slot];
-        const uint32_t j = __float_as_uint(ps.prev[slot].w);
-        const DevScene& sc = ps.jobs[j].scene;
-        const RayF r{o.x, o.y, o.z, 0.0f, d.x, d.y, d.z, __builtin_huge_valf()};
-        HitF h;
-        traceRay<BLOCK, false>(sc, r, h, LdsStack<BLOCK>{stack + threadIdx.x});
-        ps.hit[slot] = make_uint4(__float_as_uint(h.t), h.object, h.prim, __float_as_uint(h.b1));
-        ps.hit_b2[slot] = h.b2;
-        const uint32_t key = materialBin(sc, h.object);
-        ps.key[slot] = key;
-        for (uint32_t b = 0; b < uint32_t(kBins); ++b) {
-            const unsigned long long m = __ballot(key == b);
-            if (m && __lane_id() == __ffsll(static_cast<long long>(m)) - 1)
-                atomicAdd(&c->bin_count[b], uint32_t(__popcll(m)));
+    const uint32_t n = ps.ctr->n_active;
+    WaveCounts<kBins> wc;
+    for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
+        const uint32_t i = base + threadIdx.x;
+        uint32_t flags = 0;
+        if (i < n) {
+            const uint32_t slot = ps.active[i];
+            const float4 o = ps.org[slot], d = ps.dir[slot];
+            const uint32_t j = __float_as_uint(ps.prev[slot].w);
+            const DevScene& sc = ps.jobs[j].scene;
+            const RayF r{o.x, o.y, o.z, 0.0f, d.x, d.y, d.z, __builtin_huge_valf()};
+            HitF h;
+            traceRay<BLOCK, false>(sc, r, h, LdsStack<BLOCK>{stack + threadIdx.x});
+            ps.hit[slot] = make_uint4(__float_as_uint(h.t), h.object, h.prim, __float_as_uint(h.b1));
+            ps.hit_b2[slot] = h.b2;
+            flags = 1u << materialBin(sc, h.object);
+            ps.key[i] = flags;
         }
+        wc.add(flags);
     }
+    storeBlockCounts<BLOCK, kBins>(wc, ps.block_count);
 }
 
-__global__ void kBinPlan(PathState ps)
+__global__ __launch_bounds__(1024) void kPlanBins(PathState ps, uint32_t grid)
 {
-    Counters* c = ps.ctr;
-    uint32_t off = 0;
-    for (int b = 0; b < kBins; ++b) {
-        c->bin_off[b] = off;
-        c->bin_fill[b] = 0;
-        off += c->bin_count[b];
+    __shared__ uint32_t totals[kBins];
+    scanBlockCounts<kBins>(ps.block_count, ps.block_off, grid, true, totals);
+    if (threadIdx.x == 0) {
+        Counters* c = ps.ctr;
+        c->extend_rays += c->n_active;
+        c->n_in = c->n_active;
+        for (int b = 0; b < kBins; ++b) c->bin_count[b] = totals[b];
     }
-    c->extend_rays += c->n_active;
 }
 
 // Counting sort of the extended paths by material, so waves of the shade kernel
 // mostly run one material's code.
-__global__ void kScatter(PathState ps)
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void kScatterBins(PathState ps)
 {
-    Counters* c = ps.ctr;
-    const uint32_t n = c->n_active;
-    for (uint32_t i = gridStart(); i < n; i += gridStride()) {
-        const uint32_t slot = ps.active[i];
-        const uint32_t key = ps.key[slot];
-        uint32_t pos = 0;
-        for (uint32_t b = 0; b < uint32_t(kBins); ++b) {
-            const uint32_t p = waveAppend(&c->bin_fill[b], key == b);
-            if (key == b) pos = c->bin_off[b] + p;
+    const uint32_t n = ps.ctr->n_in;
+    uint32_t run[kBins];
+#pragma unroll
+    for (int b = 0; b < kBins; ++b) run[b] = ps.block_off[blockIdx.x * 8 + b];
+    for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t flags = i < n ? ps.key[i] : 0u;
+        uint32_t rank[kBins], total[kBins];
+        blockRanks<BLOCK, kBins>(flags, rank, total);
+        if (i < n) {
+            uint32_t pos = 0;
+#pragma unroll
+            for (int b = 0; b < kBins; ++b)
+                if ((flags >> b) & 1u) pos = run[b] + rank[b];
+            ps.binned[pos] = ps.active[i];
         }
-        ps.binned[pos] = slot;
+#pragma unroll
+        for (int b = 0; b < kBins; ++b) run[b] += total[b];
     }
-}
-
-__global__ void kPrepShade(PathState ps)
-{
-    Counters* c = ps.ctr;
-    c->n_in = c->n_active;
-    c->n_active = 0;
-    c->n_shadow = 0;
-    c->n_term = 0;
 }
 
 // ------------------------------------------------------------------- shade
@@ -347,13 +445,11 @@ __device__ __forceinline__ bool sampleLight(const DevLight& l, F3 p, float ux, f
 // One bounce of PathEstimator::radiance (reference path_estimator.cpp:153-185)
 // for every extended path: emission with MIS, next-event estimation (the shadow
 // ray is deferred to kShadow), BSDF sampling and Russian roulette.
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void kShade(PathState ps)
+constexpr uint32_t kQueueActive = 1u, kQueueShadow = 2u, kQueueTerm = 4u;
+
+// Shades the path in `slot` and returns the queues it joins.
+__device__ __forceinline__ uint32_t shadePath(const PathState& ps, uint32_t slot)
 {
-    Counters* c = ps.ctr;
-    const uint32_t n = c->n_in;
-    for (uint32_t i = gridStart(); i < n; i += gridStride()) {
-        const uint32_t slot = ps.binned[i];
         const float4 o4 = ps.org[slot], d4 = ps.dir[slot], t4 = ps.thr[slot], r4 = ps.rad[slot],
                      p4 = ps.prev[slot];
         const uint32_t j = __float_as_uint(p4.w);
@@ -462,12 +558,59 @@ __global__ __launch_bounds__(BLOCK) void kShade(PathState ps)
             }
         }
         ps.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
-        const uint32_t pa = waveAppend(&c->n_active, alive);
-        if (alive) ps.active[pa] = slot;
-        const uint32_t psh = waveAppend(&c->n_shadow, shadow);
-        if (shadow) ps.shadowq[psh] = slot;
-        const uint32_t pt = waveAppend(&c->n_term, !alive);
-        if (!alive) ps.termq[pt] = slot;
+        return (alive ? kQueueActive : kQueueTerm) | (shadow ? kQueueShadow : 0u);
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void kShade(PathState ps)
+{
+    const uint32_t n = ps.ctr->n_in;
+    WaveCounts<3> wc;
+    for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
+        const uint32_t i = base + threadIdx.x;
+        uint32_t flags = 0;
+        if (i < n) {
+            flags = shadePath(ps, ps.binned[i]);
+            ps.key[i] = flags;
+        }
+        wc.add(flags);
+    }
+    storeBlockCounts<BLOCK, 3>(wc, ps.block_count);
+}
+
+__global__ __launch_bounds__(1024) void kPlanShade(PathState ps, uint32_t grid)
+{
+    __shared__ uint32_t totals[3];
+    scanBlockCounts<3>(ps.block_count, ps.block_off, grid, false, totals);
+    if (threadIdx.x == 0) {
+        Counters* c = ps.ctr;
+        c->n_active = totals[0];
+        c->n_shadow = totals[1];
+        c->n_term = totals[2];
+    }
+}
+
+// Writes the shaded paths into the extension, shadow and termination queues.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void kScatterShade(PathState ps)
+{
+    const uint32_t n = ps.ctr->n_in;
+    uint32_t run[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) run[q] = ps.block_off[blockIdx.x * 8 + q];
+    for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t flags = i < n ? ps.key[i] : 0u;
+        uint32_t rank[3], total[3];
+        blockRanks<BLOCK, 3>(flags, rank, total);
+        if (i < n) {
+            const uint32_t slot = ps.binned[i];
+            if (flags & kQueueActive) ps.active[run[0] + rank[0]] = slot;
+            if (flags & kQueueShadow) ps.shadowq[run[1] + rank[1]] = slot;
+            if (flags & kQueueTerm) ps.termq[run[2] + rank[2]] = slot;
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) run[q] += total[q];
     }
 }
 
@@ -515,14 +658,26 @@ __global__ void kFinish(PathState ps)
         const uint32_t slot = ps.termq[i];
         const uint32_t j = __float_as_uint(ps.prev[slot].w);
         const DevJob& job = ps.jobs[j];
-        if (!job.cancelled) {
+        const bool counts = !job.cancelled;
+        if (counts) {
             const float4 r = ps.rad[slot];
             unsigned long long* a = job.accum + 4 * size_t(ps.pixel[slot]);
             atomicAdd(a + 0, toFixed(r.x));
             atomicAdd(a + 1, toFixed(r.y));
             atomicAdd(a + 2, toFixed(r.z));
             atomicAdd(a + 3, 1ull);
-            atomicAdd(job.done, 1ull);
+        }
+        // The per-job finished-sample counter is one address: add once per wave
+        // when the wave's paths share a job (the usual case), else per lane.
+        const unsigned long long live = __ballot(counts);
+        if (live) {
+            const int leader = __ffsll(static_cast<long long>(live)) - 1;
+            const uint32_t j0 = __shfl(j, leader);
+            if (__ballot(counts && j != j0) == 0) {
+                if (__lane_id() == leader) atomicAdd(ps.jobs[j0].done, static_cast<unsigned long long>(__popcll(live)));
+            } else if (counts) {
+                atomicAdd(job.done, 1ull);
+            }
         }
         ps.freeq[free0 + i] = slot;
     }
@@ -567,10 +722,11 @@ hipError_t bounce(const PathState& ps, uint32_t grid, hipStream_t s)
     hipLaunchKernelGGL(kGenerate, dim3(grid), dim3(BLOCK), 0, s, ps);
     hipLaunchKernelGGL(kPostGenerate, dim3(1), dim3(1), 0, s, ps);
     hipLaunchKernelGGL((kExtend<BLOCK, STACK>), dim3(grid), dim3(BLOCK), 0, s, ps);
-    hipLaunchKernelGGL(kBinPlan, dim3(1), dim3(1), 0, s, ps);
-    hipLaunchKernelGGL(kScatter, dim3(grid), dim3(BLOCK), 0, s, ps);
-    hipLaunchKernelGGL(kPrepShade, dim3(1), dim3(1), 0, s, ps);
+    hipLaunchKernelGGL(kPlanBins, dim3(1), dim3(1024), 0, s, ps, grid);
+    hipLaunchKernelGGL((kScatterBins<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     hipLaunchKernelGGL((kShade<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
+    hipLaunchKernelGGL(kPlanShade, dim3(1), dim3(1024), 0, s, ps, grid);
+    hipLaunchKernelGGL((kScatterShade<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     hipLaunchKernelGGL((kShadow<BLOCK, STACK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     hipLaunchKernelGGL(kFinish, dim3(grid), dim3(BLOCK), 0, s, ps);
     hipLaunchKernelGGL(kPost, dim3(1), dim3(1), 0, s, ps);

@@ -33,9 +33,7 @@ struct alignas(16) Counters {
     uint32_t n_term;
     uint32_t n_free;
     uint32_t n_new;        // paths generate() creates this round
-    uint32_t bin_count[8];
-    uint32_t bin_fill[8];
-    uint32_t bin_off[8];
+    uint32_t bin_count[8]; // paths per material bin in the last bounce
     uint32_t n_jobs;       // entries of the job table in use
     uint32_t pad;
     unsigned long long work_next;
@@ -47,6 +45,7 @@ struct alignas(16) Counters {
 };
 
 constexpr int kBins = 5; // miss, diffuse, conductor, dielectric, mix
+constexpr uint32_t kMaxGrid = 8192; // blocks of the grid-stride queue kernels
 
 // Path state, structure of arrays over `capacity` slots.
 struct PathState {
@@ -59,7 +58,9 @@ struct PathState {
     uint32_t* pixel;
     uint4* hit;     // t bits, object, prim, b1 bits
     float* hit_b2;
-    uint32_t* key;  // material bin of the hit
+    uint32_t* key;  // per queue position: the output queues of the item (bit mask)
+    uint32_t* block_count; // per block of a queue kernel: items per output queue (8 per block)
+    uint32_t* block_off;   // per block: first output position per queue
     float4* sh_org; // xyz origin, w tmax
     float4* sh_dir; // xyz direction
     float4* sh_c;   // xyz contribution if unoccluded
