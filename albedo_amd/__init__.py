@@ -195,6 +195,14 @@ class Server:
         self.shapes[jid.value] = (height, width)
         return jid.value
 
+    @staticmethod
+    def footprint(scene, camera, width, height, samples_per_axis, max_depth=64, roulette_start=3, seed=0):
+        """HBM bytes the job holds while admitted (what the budget is charged)."""
+        d = Server.job_desc(scene, camera, width, height, samples_per_axis, max_depth, roulette_start, seed)
+        b = ctypes.c_uint64()
+        check(lib().ab_job_footprint(ctypes.byref(d), ctypes.byref(b)))
+        return b.value
+
     def _status(self, st):
         return {"state": st.state, "samples_done": st.samples_done, "samples_total": st.samples_total,
                 "submit_ms": st.submit_ms, "start_ms": st.start_ms, "finish_ms": st.finish_ms}

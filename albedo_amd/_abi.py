@@ -142,6 +142,7 @@ PROTOTYPES = {
     "ab_server_start": (_S, [_P, ctypes.c_uint64]),
     "ab_server_stop": (_S, [_P]),
     "ab_job_submit": (_S, [_P, ctypes.POINTER(JobDesc), ctypes.POINTER(ctypes.c_uint64)]),
+    "ab_job_footprint": (_S, [ctypes.POINTER(JobDesc), ctypes.POINTER(ctypes.c_uint64)]),
     "ab_job_poll": (_S, [_P, ctypes.c_uint64, ctypes.POINTER(JobStatus), _P]),
     "ab_job_wait": (_S, [_P, ctypes.c_uint64, ctypes.POINTER(JobStatus), _P]),
     "ab_job_cancel": (_S, [_P, ctypes.c_uint64]),

@@ -23,6 +23,9 @@ bool finite3(const double* p) { return std::isfinite(p[0]) && std::isfinite(p[1]
 // 2 jitter draws + at most 7 per bounce.
 constexpr uint32_t kMaxDepthLimit = uint32_t((1024 - 2) / 7);
 
+// Samples of one pixel generated back to back (a pass of the work order).
+constexpr uint32_t kPixelGroup = 16;
+
 // Blocks of the grid-stride queue kernels: the configured count, or by default
 // enough for one pass over a full queue up to 2048 blocks (8 per CU).
 uint32_t gridFor(uint32_t configured, uint32_t capacity, uint32_t block)
@@ -99,6 +102,7 @@ ab_status prepareJob(const JobSpec& spec, DevJob& dj)
     dj.spa = p.samples_per_axis;
     dj.max_depth = p.max_depth;
     dj.roulette_start = p.roulette_start;
+    dj.group = std::min<uint32_t>(p.samples_per_axis * p.samples_per_axis, kPixelGroup);
     dj.seed = p.seed;
     dj.work_count = uint64_t(p.width) * p.height * p.samples_per_axis * p.samples_per_axis;
     return AB_OK;
@@ -143,6 +147,7 @@ ab_status Engine::reserve(uint32_t capacity, uint32_t max_jobs, hipStream_t)
                     {reinterpret_cast<void**>(&p.sh_c), C * 16},     {reinterpret_cast<void**>(&p.active), C * 4},
                     {reinterpret_cast<void**>(&p.binned), C * 4},    {reinterpret_cast<void**>(&p.shadowq), C * 4},
                     {reinterpret_cast<void**>(&p.termq), C * 4},     {reinterpret_cast<void**>(&p.freeq), C * 4},
+                    {reinterpret_cast<void**>(&p.active_next), C * 4},
                     {reinterpret_cast<void**>(&p.block_count), size_t(kMaxGrid) * 8 * 4},
                     {reinterpret_cast<void**>(&p.block_off), size_t(kMaxGrid) * 8 * 4},
                     {reinterpret_cast<void**>(&p.ctr), sizeof(Counters)},
@@ -162,13 +167,13 @@ ab_status Engine::reserve(uint32_t capacity, uint32_t max_jobs, hipStream_t)
     return AB_OK;
 }
 
-ab_status Engine::bounceOnce(const LaunchShape& shape)
+ab_status Engine::bouncePair(const LaunchShape& shape)
 {
     if (use_graph_ && graph_) {
         HIP_TRY(hipGraphLaunch(graph_, stream_), "hipGraphLaunch");
         return AB_OK;
     }
-    HIP_TRY(launchBounce(ps_, shape, stream_), "bounce launch");
+    HIP_TRY(launchBouncePair(ps_, shape, stream_), "bounce launch");
     return AB_OK;
 }
 
@@ -191,7 +196,7 @@ ab_status Engine::runUntilDone(const LaunchShape& shape, bool use_graph, Counter
     if (use_graph) {
         hipGraph_t g = nullptr;
         HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-        const hipError_t le = launchBounce(ps_, shape, stream_);
+        const hipError_t le = launchBouncePair(ps_, shape, stream_);
         const hipError_t ce = hipStreamEndCapture(stream_, &g);
         if (le != hipSuccess) return hipFail(le, "bounce capture");
         HIP_TRY(ce, "hipStreamEndCapture");
@@ -208,18 +213,18 @@ ab_status Engine::runUntilDone(const LaunchShape& shape, bool use_graph, Counter
     for (const DevJob& j : jobs_) max_depth = std::max<uint64_t>(max_depth, j.max_depth);
     const uint64_t limit = 4 * (max_depth + 2) * (work_total_ / capacity_ + 2) + 64;
     uint64_t bounces = 0;
-    uint32_t per_check = 4;
+    uint32_t pairs_per_check = 2;
     while (true) {
         if (bounces > limit) return fail(AB_DEVICE_ERROR, "render did not finish within its bounce bound");
-        bounces += per_check;
-        for (uint32_t k = 0; k < per_check; ++k) {
-            const ab_status st = bounceOnce(shape);
+        bounces += 2 * pairs_per_check;
+        for (uint32_t k = 0; k < pairs_per_check; ++k) {
+            const ab_status st = bouncePair(shape);
             if (st != AB_OK) return st;
         }
         const ab_status st = readCounters(c);
         if (st != AB_OK) return st;
         if (idle(c)) return AB_OK;
-        per_check = std::min<uint32_t>(per_check * 2, 64);
+        pairs_per_check = std::min<uint32_t>(pairs_per_check * 2, 32);
     }
 }
 
@@ -231,13 +236,15 @@ ab_status Engine::renderBatch(const std::vector<JobSpec>& specs, const std::vect
     uint32_t stack = 0;
     uint64_t total = 0;
     size_t accum_bytes = 0;
+    std::vector<JobBuffers> bufs(specs.size());
     for (size_t i = 0; i < specs.size(); ++i) {
         const ab_status st = prepareJob(specs[i], jobs_[i]);
         if (st != AB_OK) return st;
         jobs_[i].work_begin = total;
         total += jobs_[i].work_count;
         stack = std::max(stack, jobs_[i].scene.stack_depth);
-        accum_bytes += align256(size_t(jobs_[i].width) * jobs_[i].height * 32 + 8);
+        bufs[i] = jobBuffers(jobs_[i]);
+        accum_bytes += bufs[i].total;
     }
     if (stack > 64) return fail(AB_INVALID_ARGUMENT, "scene BVH too deep for the wavefront engine (stack > 64)");
     const uint32_t capacity = std::max<uint32_t>(cfg.paths_in_flight, 1024);
@@ -247,10 +254,11 @@ ab_status Engine::renderBatch(const std::vector<JobSpec>& specs, const std::vect
     HIP_TRY(hipMallocAsync(&acc, accum_bytes, stream_), "hipMalloc(accumulators)");
     HIP_TRY(hipMemsetAsync(acc, 0, accum_bytes, stream_), "hipMemset(accumulators)");
     size_t off = 0;
-    for (DevJob& j : jobs_) {
-        j.accum = reinterpret_cast<unsigned long long*>(static_cast<char*>(acc) + off);
-        j.done = j.accum + size_t(j.width) * j.height * 4;
-        off += align256(size_t(j.width) * j.height * 32 + 8);
+    for (size_t i = 0; i < jobs_.size(); ++i) {
+        char* base = static_cast<char*>(acc) + off;
+        jobs_[i].accum = reinterpret_cast<unsigned long long*>(base);
+        jobs_[i].done = reinterpret_cast<unsigned long long*>(base + bufs[i].done_offset);
+        off += bufs[i].total;
     }
     work_total_ = total;
     HIP_TRY(hipMemcpyAsync(d_jobs_, jobs_.data(), sizeof(DevJob) * jobs_.size(), hipMemcpyHostToDevice, stream_),
@@ -266,23 +274,17 @@ ab_status Engine::renderBatch(const std::vector<JobSpec>& specs, const std::vect
     Counters c{};
     st = runUntilDone(shape, cfg.use_graph != 0, c);
     (void)hipEventRecord(e1, stream_);
-    std::vector<unsigned long long> host;
     for (size_t i = 0; st == AB_OK && i < jobs_.size(); ++i) {
         const DevJob& j = jobs_[i];
         const size_t npx = size_t(j.width) * j.height;
-        host.resize(npx * 4);
-        const hipError_t e = hipMemcpyAsync(host.data(), j.accum, npx * 32, hipMemcpyDeviceToHost, stream_);
-        if (e != hipSuccess) {
-            st = hipFail(e, "hipMemcpy(image)");
-            break;
-        }
-        (void)hipStreamSynchronize(stream_);
-        float* dst = out[i];
-        for (size_t px = 0; px < npx; ++px) {
-            const double n = double(host[4 * px + 3]);
-            for (int ch = 0; ch < 3; ++ch)
-                dst[3 * px + ch] = n > 0 ? float(double(host[4 * px + ch]) / 268435456.0 / n) : 0.0f;
-        }
+        float* mean = reinterpret_cast<float*>(reinterpret_cast<char*>(j.accum) + bufs[i].mean_offset);
+        hipError_t e = launchResolve(j.accum, npx, mean, stream_);
+        if (e == hipSuccess) e = hipMemcpyAsync(out[i], mean, npx * 12, hipMemcpyDeviceToHost, stream_);
+        if (e != hipSuccess) st = hipFail(e, "image readback");
+    }
+    if (st == AB_OK) {
+        const hipError_t e = hipStreamSynchronize(stream_);
+        if (e != hipSuccess) st = hipFail(e, "hipStreamSynchronize(images)");
     }
     (void)hipEventSynchronize(e1);
     float ms = 0;
@@ -320,7 +322,7 @@ ab_status Engine::beginStream(const ab_launch_config& cfg, uint32_t max_jobs)
     if (use_graph_) {
         hipGraph_t g = nullptr;
         HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-        const hipError_t le = launchBounce(ps_, shape_, stream_);
+        const hipError_t le = launchBouncePair(ps_, shape_, stream_);
         const hipError_t ce = hipStreamEndCapture(stream_, &g);
         if (le != hipSuccess) return hipFail(le, "bounce capture");
         HIP_TRY(ce, "hipStreamEndCapture");
@@ -363,8 +365,8 @@ ab_status Engine::cancelJob(uint32_t index)
 
 ab_status Engine::runBounces(uint32_t count)
 {
-    for (uint32_t k = 0; k < count; ++k) {
-        const ab_status st = bounceOnce(shape_);
+    for (uint32_t k = 0; k < count; k += 2) {
+        const ab_status st = bouncePair(shape_);
         if (st != AB_OK) return st;
     }
     return AB_OK;

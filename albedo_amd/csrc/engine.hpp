@@ -20,6 +20,25 @@ struct JobSpec {
 // perspective_camera.cpp:24-110) and fills the FP32 camera frame of `dj`.
 ab_status prepareJob(const JobSpec& spec, DevJob& dj);
 
+// Device memory of one job, in one allocation: the per-pixel accumulators, the
+// finished-sample counter and the resolved mean image (float RGB per pixel).
+struct JobBuffers {
+    size_t done_offset;
+    size_t mean_offset;
+    size_t total;
+};
+
+inline JobBuffers jobBuffers(const DevJob& j)
+{
+    const auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t npx = size_t(j.width) * j.height;
+    JobBuffers b;
+    b.done_offset = align(npx * 32);
+    b.mean_offset = b.done_offset + kDoneBytes;
+    b.total = b.mean_offset + align(npx * 12);
+    return b;
+}
+
 class Engine {
 public:
     explicit Engine(ab_context* ctx) : ctx_(ctx) {}
@@ -41,7 +60,7 @@ public:
     ab_status appendJob(const DevJob& prepared, unsigned long long* accum, unsigned long long* done,
                         uint32_t* index);
     ab_status cancelJob(uint32_t index);
-    ab_status runBounces(uint32_t count);
+    ab_status runBounces(uint32_t count); // rounded up to an even count
     ab_status resetTable();
     bool tableFull() const { return jobs_.size() >= max_jobs_; }
     size_t tableSize() const { return jobs_.size(); }
@@ -52,7 +71,7 @@ public:
 
 private:
     ab_status runUntilDone(const LaunchShape& shape, bool use_graph, Counters& final_counters);
-    ab_status bounceOnce(const LaunchShape& shape);
+    ab_status bouncePair(const LaunchShape& shape);
     void freeAll();
 
     ab_context* ctx_;

@@ -27,8 +27,6 @@ namespace {
 
 enum JobState : uint32_t { kQueued = 0, kRunning = 1, kDone = 2, kCancelled = 3, kFailed = 4 };
 
-size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
-
 struct ServerJob {
     uint64_t id{0};
     JobSpec spec;
@@ -38,19 +36,23 @@ struct ServerJob {
     uint64_t done{0};
     double submit_ms{0}, start_ms{0}, finish_ms{0};
     size_t footprint{0};
-    void* dev{nullptr}; // accumulators (4 u64 per pixel) + done counter
+    JobBuffers buf{};
+    void* dev{nullptr}; // accumulators, done counter and mean image (JobBuffers layout)
     uint32_t engine_index{0};
     bool cancel_requested{false};
     std::vector<float> result;
 };
 
-void meanFromSums(const std::vector<unsigned long long>& sums, size_t npx, float* out)
+// Resolves a job's accumulators into its mean image on `s` and copies it to host.
+hipError_t readMean(const ServerJob& j, float* out, hipStream_t s)
 {
-    for (size_t px = 0; px < npx; ++px) {
-        const double n = double(sums[4 * px + 3]);
-        for (int ch = 0; ch < 3; ++ch)
-            out[3 * px + ch] = n > 0 ? float(double(sums[4 * px + ch]) / 268435456.0 / n) : 0.0f;
-    }
+    const size_t npx = size_t(j.prepared.width) * j.prepared.height;
+    auto* acc = static_cast<unsigned long long*>(j.dev);
+    float* mean = reinterpret_cast<float*>(static_cast<char*>(j.dev) + j.buf.mean_offset);
+    hipError_t e = launchResolve(acc, npx, mean, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, mean, npx * 12, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e;
 }
 
 } // namespace
@@ -80,7 +82,6 @@ struct Server {
 
     Server(ab_context* c, uint64_t b) : ctx(c), budget(b), engine(c) {}
 
-    size_t footprintOf(const DevJob& j) const { return align256(size_t(j.width) * j.height * 32 + 8); }
 
     void loop();
     ab_status admitLocked();
@@ -106,9 +107,9 @@ ab_status Server::admitLocked()
             return hipFail(e, "hipMemset(job)");
         }
         auto* acc = static_cast<unsigned long long*>(dev);
-        const size_t npx = size_t(j.prepared.width) * j.prepared.height;
+        auto* done = reinterpret_cast<unsigned long long*>(static_cast<char*>(dev) + j.buf.done_offset);
         uint32_t index = 0;
-        const ab_status st = engine.appendJob(j.prepared, acc, acc + 4 * npx, &index);
+        const ab_status st = engine.appendJob(j.prepared, acc, done, &index);
         if (st != AB_OK) {
             (void)hipFree(dev);
             j.state = kFailed;
@@ -136,12 +137,15 @@ void Server::retireFinished()
         std::lock_guard<std::mutex> lock(mu);
         for (uint64_t id : running) live.push_back(jobs[id]);
     }
+    std::vector<unsigned long long> lanes(kDoneBytes / 8);
     for (auto& jp : live) {
         ServerJob& j = *jp;
         const size_t npx = size_t(j.prepared.width) * j.prepared.height;
-        unsigned long long d = 0;
-        if (hipMemcpy(&d, static_cast<unsigned long long*>(j.dev) + 4 * npx, 8, hipMemcpyDeviceToHost) != hipSuccess)
+        if (hipMemcpy(lanes.data(), static_cast<char*>(j.dev) + j.buf.done_offset, kDoneBytes,
+                      hipMemcpyDeviceToHost) != hipSuccess)
             continue;
+        unsigned long long d = 0;
+        for (uint32_t k = 0; k < kDoneLanes; ++k) d += lanes[size_t(k) * kDoneStride];
         bool cancel = false;
         {
             std::lock_guard<std::mutex> lock(mu);
@@ -161,12 +165,14 @@ void Server::retireFinished()
             continue;
         }
         if (d < j.total) continue;
-        std::vector<unsigned long long> sums(npx * 4);
-        const hipError_t e = hipMemcpy(sums.data(), j.dev, npx * 32, hipMemcpyDeviceToHost);
+        // The engine stream is idle here (readCounters synchronised it), so every
+        // sample of the job has landed. The lock keeps polls off the job's mean
+        // buffer meanwhile.
         std::lock_guard<std::mutex> lock(mu);
+        std::vector<float> result(npx * 3);
+        const hipError_t e = readMean(j, result.data(), engine.stream());
         if (e == hipSuccess) {
-            j.result.resize(npx * 3);
-            meanFromSums(sums, npx, j.result.data());
+            j.result = std::move(result);
             j.state = kDone;
         } else {
             j.state = kFailed;
@@ -285,6 +291,7 @@ ab_status ab_server_stop(ab_context* ctx)
     }
     s->cv.notify_all();
     if (s->thread.joinable()) s->thread.join();
+    if (s->poll_stream) (void)hipStreamDestroy(s->poll_stream);
     ctx->server = nullptr;
     delete s;
     return AB_OK;
@@ -304,7 +311,8 @@ ab_status ab_job_submit(ab_context* ctx, const ab_job_desc* job, uint64_t* job_i
     if (j->prepared.scene.stack_depth > 64)
         return fail(AB_INVALID_ARGUMENT, "scene BVH too deep for the wavefront engine (stack > 64)");
     j->total = j->prepared.work_count;
-    j->footprint = s->footprintOf(j->prepared);
+    j->buf = jobBuffers(j->prepared);
+    j->footprint = j->buf.total;
     std::lock_guard<std::mutex> lock(s->mu);
     if (j->footprint > s->budget) {
         ++s->jobs_rejected;
@@ -317,6 +325,16 @@ ab_status ab_job_submit(ab_context* ctx, const ab_job_desc* job, uint64_t* job_i
     s->pending.push_back(j->id);
     *job_id = j->id;
     s->cv.notify_all();
+    return AB_OK;
+}
+
+ab_status ab_job_footprint(const ab_job_desc* job, uint64_t* bytes)
+{
+    if (!job || !bytes) return fail(AB_INVALID_ARGUMENT, "ab_job_footprint: null argument");
+    DevJob dj{};
+    const ab_status st = prepareJob(JobSpec{job->scene, job->camera, job->params}, dj);
+    if (st != AB_OK) return st;
+    *bytes = jobBuffers(dj).total;
     return AB_OK;
 }
 
@@ -346,11 +364,14 @@ ab_status ab_job_poll(ab_context* ctx, uint64_t id, ab_job_status* status, float
             std::copy(j.result.begin(), j.result.end(), rgb);
         } else if (j.state == kRunning && j.dev) {
             // Progressive snapshot of sums and counts (the reference keeps both,
-            // image/pixel_buffer.hpp:54-57).
-            std::vector<unsigned long long> sums(npx * 4);
-            if (hipMemcpy(sums.data(), j.dev, npx * 32, hipMemcpyDeviceToHost) != hipSuccess)
-                return fail(AB_DEVICE_ERROR, "ab_job_poll: snapshot copy failed");
-            meanFromSums(sums, npx, rgb);
+            // image/pixel_buffer.hpp:54-57), resolved while the engine runs. Polls
+            // hold the server lock, so one at a time uses the job's mean buffer;
+            // the final resolve happens after the job leaves kRunning.
+            (void)hipSetDevice(ctx->device);
+            if (!s->poll_stream && hipStreamCreateWithFlags(&s->poll_stream, hipStreamNonBlocking) != hipSuccess)
+                return fail(AB_DEVICE_ERROR, "ab_job_poll: stream creation failed");
+            const hipError_t e = readMean(j, rgb, s->poll_stream);
+            if (e != hipSuccess) return hipFail(e, "ab_job_poll: snapshot");
         } else {
             std::fill(rgb, rgb + npx * 3, 0.0f);
         }

@@ -44,9 +44,11 @@ __device__ __forceinline__ uint32_t gridStride() { return gridDim.x * blockDim.x
 // ---------------------------------------------------------------- generate
 
 // Camera rays for the next n_new samples (reference render_image.cpp:27-44 and
-// PerspectiveCamera::generateRay, perspective_camera.cpp:74-87). Work index w of
-// a job maps to sample s = w / (W H) and pixel w mod (W H), so every pixel gets
-// its first sample before any gets its second.
+// PerspectiveCamera::generateRay, perspective_camera.cpp:74-87). A job's samples
+// are taken in passes of `group` samples per pixel: pass g covers samples
+// [g G, g G + G) of every pixel (the last pass fewer), pixel-major inside the
+// pass. Samples of one pixel are then neighbours in the queues, so kFinish can
+// sum them within a wave before touching the pixel.
 __global__ void kGenerate(PathState ps)
 {
     const Counters* c = ps.ctr;
@@ -59,8 +61,14 @@ __global__ void kGenerate(PathState ps)
         const DevJob& job = ps.jobs[j];
         const unsigned long long local = w - job.work_begin;
         const unsigned long long npx = uint64_t(job.width) * job.height;
-        const uint32_t s = uint32_t(local / npx);
-        const uint32_t pix = uint32_t(local - uint64_t(s) * npx);
+        const uint32_t group = job.group, spp = job.spa * job.spa;
+        const unsigned long long span = npx * group;
+        const uint32_t pass = uint32_t(local / span);
+        const unsigned long long r = local - uint64_t(pass) * span;
+        const uint32_t first = pass * group;
+        const uint32_t in_pass = min(group, spp - first);
+        const uint32_t pix = uint32_t(r / in_pass);
+        const uint32_t s = first + uint32_t(r - uint64_t(pix) * in_pass);
         const uint32_t x = pix % job.width, y = pix / job.width;
         Rng rng = sampleRng(job.seed, pix, s);
         const uint32_t sx = s % job.spa, sy = s / job.spa;
@@ -142,41 +150,66 @@ __device__ __forceinline__ void storeBlockCounts(const WaveCounts<NQ>& wc, uint3
 
 // Exclusive scan of block_count over `grid` blocks for each of NQ queues into
 // block_off. concat: queues share one output array, queue q after queues < q.
-// Totals go to totals[q]. One block of 1024 threads.
+// Totals go to totals[q]. One block of 1024 threads: each thread sums a run of
+// consecutive blocks, waves scan their threads' sums with shuffles, and the 16
+// wave totals are combined through LDS.
 template <int NQ>
 __device__ __forceinline__ void scanBlockCounts(const uint32_t* block_count, uint32_t* block_off, uint32_t grid,
                                                 bool concat, uint32_t* totals)
 {
-    __shared__ uint32_t part[1024];
-    __shared__ uint32_t base;
-    const uint32_t t = threadIdx.x;
+    constexpr uint32_t kWaves = 1024 / 64;
+    constexpr uint32_t kPer = kMaxGrid / 1024;
+    __shared__ uint32_t wsum[kWaves][NQ];
+    const uint32_t t = threadIdx.x, lane = __lane_id(), wave = t >> 6;
     const uint32_t per = (grid + 1023) / 1024;
-    const uint32_t b0 = t * per, b1 = min(grid, b0 + per);
-    if (t == 0) base = 0;
-    for (int q = 0; q < NQ; ++q) {
-        uint32_t local = 0;
-        for (uint32_t b = b0; b < b1; ++b) local += block_count[b * 8 + q];
-        part[t] = local;
-        __syncthreads();
-        if (t == 0) {
-            uint32_t run = 0;
-            for (uint32_t k = 0; k < 1024; ++k) {
-                const uint32_t v = part[k];
-                part[k] = run;
-                run += v;
-            }
-            totals[q] = run;
+    const uint32_t b0 = min(grid, t * per), b1 = min(grid, b0 + per);
+    // All of the thread's counts are loaded at once (unused entries read block 0
+    // and count as zero), so the loads overlap instead of forming a chain.
+    uint32_t cnt[NQ][kPer];
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        const bool ok = b0 + k < b1;
+        const uint32_t b = ok ? b0 + k : 0u;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const uint32_t x = block_count[b * 8 + q];
+            cnt[q][k] = ok ? x : 0u;
         }
-        __syncthreads();
-        uint32_t run = (concat ? base : 0) + part[t];
-        for (uint32_t b = b0; b < b1; ++b) {
-            block_off[b * 8 + q] = run;
-            run += block_count[b * 8 + q];
-        }
-        __syncthreads();
-        if (t == 0 && concat) base += totals[q];
-        __syncthreads();
     }
+    uint32_t local[NQ], incl[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        uint32_t v = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) v += cnt[q][k];
+        local[q] = v;
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            const uint32_t u = __shfl_up(v, off);
+            if (lane >= off) v += u;
+        }
+        incl[q] = v;
+        if (lane == 63) wsum[wave][q] = v;
+    }
+    __syncthreads();
+    uint32_t base = 0;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        uint32_t before = 0, all = 0;
+        for (uint32_t w = 0; w < kWaves; ++w) {
+            const uint32_t v = wsum[w][q];
+            before += w < wave ? v : 0;
+            all += v;
+        }
+        uint32_t run = (concat ? base : 0) + before + incl[q] - local[q];
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {
+            if (b0 + k < b1) block_off[(b0 + k) * 8 + q] = run;
+            run += cnt[q][k];
+        }
+        if (t == 0) totals[q] = all;
+        base += all;
+    }
+    __syncthreads();
 }
 
 // Rank of this lane among the block's lanes with bit q of `flags`, for one
@@ -256,7 +289,10 @@ __global__ __launch_bounds__(1024) void kPlanBins(PathState ps, uint32_t grid)
 }
 
 // Counting sort of the extended paths by material, so waves of the shade kernel
-// mostly run one material's code.
+// mostly run one material's code. Only the shade kernel reads in this order:
+// `binned` holds positions in `active`, and the queues the bounce produces keep
+// the order of `active` (generation order), so samples of one pixel stay
+// neighbours from bounce to bounce.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void kScatterBins(PathState ps)
 {
@@ -274,7 +310,7 @@ __global__ __launch_bounds__(BLOCK) void kScatterBins(PathState ps)
 #pragma unroll
             for (int b = 0; b < kBins; ++b)
                 if ((flags >> b) & 1u) pos = run[b] + rank[b];
-            ps.binned[pos] = ps.active[i];
+            ps.binned[pos] = i;
         }
 #pragma unroll
         for (int b = 0; b < kBins; ++b) run[b] += total[b];
@@ -561,19 +597,27 @@ __device__ __forceinline__ uint32_t shadePath(const PathState& ps, uint32_t slot
         return (alive ? kQueueActive : kQueueTerm) | (shadow ? kQueueShadow : 0u);
 }
 
+// Shades in material order; the queue flags go to the path's position in
+// `active`.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void kShade(PathState ps)
+{
+    const uint32_t n = ps.ctr->n_in;
+    for (uint32_t k = gridStart(); k < n; k += gridStride()) {
+        const uint32_t i = ps.binned[k];
+        ps.key[i] = shadePath(ps, ps.active[i]);
+    }
+}
+
+// Counts the shaded paths per output queue and block, in `active` order.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void kCountShade(PathState ps)
 {
     const uint32_t n = ps.ctr->n_in;
     WaveCounts<3> wc;
     for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
         const uint32_t i = base + threadIdx.x;
-        uint32_t flags = 0;
-        if (i < n) {
-            flags = shadePath(ps, ps.binned[i]);
-            ps.key[i] = flags;
-        }
-        wc.add(flags);
+        wc.add(i < n ? ps.key[i] : 0u);
     }
     storeBlockCounts<BLOCK, 3>(wc, ps.block_count);
 }
@@ -590,7 +634,8 @@ __global__ __launch_bounds__(1024) void kPlanShade(PathState ps, uint32_t grid)
     }
 }
 
-// Writes the shaded paths into the extension, shadow and termination queues.
+// Writes the shaded paths into the next extension queue and the shadow and
+// termination queues, keeping their order in `active`.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void kScatterShade(PathState ps)
 {
@@ -604,8 +649,8 @@ __global__ __launch_bounds__(BLOCK) void kScatterShade(PathState ps)
         uint32_t rank[3], total[3];
         blockRanks<BLOCK, 3>(flags, rank, total);
         if (i < n) {
-            const uint32_t slot = ps.binned[i];
-            if (flags & kQueueActive) ps.active[run[0] + rank[0]] = slot;
+            const uint32_t slot = ps.active[i];
+            if (flags & kQueueActive) ps.active_next[run[0] + rank[0]] = slot;
             if (flags & kQueueShadow) ps.shadowq[run[1] + rank[1]] = slot;
             if (flags & kQueueTerm) ps.termq[run[2] + rank[2]] = slot;
         }
@@ -650,36 +695,91 @@ __device__ __forceinline__ unsigned long long toFixed(float v)
 
 // Adds each finished path's radiance to its pixel (reference PixelBuffer::addSample,
 // image/pixel_buffer.cpp:29-35) and returns its slot to the free list.
-__global__ void kFinish(PathState ps)
+// Lanes that finish samples of the same pixel are usually neighbours (samples
+// of a pixel are generated together), so each wave first sums runs of equal
+// (job, pixel) keys and only the first lane of a run adds to the pixel: fewer
+// and far less contended atomics. Integer sums are exact, so the image is the
+// same as with one atomic add per sample.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void kFinish(PathState ps)
 {
     Counters* c = ps.ctr;
     const uint32_t n = c->n_term, free0 = c->n_free;
-    for (uint32_t i = gridStart(); i < n; i += gridStride()) {
-        const uint32_t slot = ps.termq[i];
-        const uint32_t j = __float_as_uint(ps.prev[slot].w);
-        const DevJob& job = ps.jobs[j];
-        const bool counts = !job.cancelled;
-        if (counts) {
-            const float4 r = ps.rad[slot];
-            unsigned long long* a = job.accum + 4 * size_t(ps.pixel[slot]);
-            atomicAdd(a + 0, toFixed(r.x));
-            atomicAdd(a + 1, toFixed(r.y));
-            atomicAdd(a + 2, toFixed(r.z));
-            atomicAdd(a + 3, 1ull);
+    const uint32_t lane = __lane_id();
+    for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
+        const uint32_t i = base + threadIdx.x;
+        uint32_t j = 0, pix = 0;
+        bool counts = false;
+        unsigned long long v0 = 0, v1 = 0, v2 = 0, cnt = 0;
+        if (i < n) {
+            const uint32_t slot = ps.termq[i];
+            j = __float_as_uint(ps.prev[slot].w);
+            counts = !ps.jobs[j].cancelled;
+            if (counts) {
+                const float4 r = ps.rad[slot];
+                v0 = toFixed(r.x);
+                v1 = toFixed(r.y);
+                v2 = toFixed(r.z);
+                cnt = 1;
+                pix = ps.pixel[slot];
+            }
+            ps.freeq[free0 + i] = slot;
         }
-        // The per-job finished-sample counter is one address: add once per wave
-        // when the wave's paths share a job (the usual case), else per lane.
+        const unsigned long long key = counts ? (static_cast<unsigned long long>(j) << 32) | pix : ~0ull;
+        const unsigned long long up = __shfl_up(key, 1u);
+        const bool head = lane == 0 || up != key;
+        const unsigned long long heads = __ballot(head);
+        if (heads != ~0ull) {
+            // Segmented suffix sums: run ids increase along the wave, so equal
+            // ids mean the same contiguous run.
+            const uint32_t run = uint32_t(__popcll(heads & ((2ull << lane) - 1ull)));
+            for (uint32_t off = 1; off < 64; off <<= 1) {
+                const uint32_t r2 = __shfl_down(run, off);
+                const unsigned long long a0 = __shfl_down(v0, off), a1 = __shfl_down(v1, off),
+                                         a2 = __shfl_down(v2, off), a3 = __shfl_down(cnt, off);
+                if (lane + off < 64 && r2 == run) {
+                    v0 += a0;
+                    v1 += a1;
+                    v2 += a2;
+                    cnt += a3;
+                }
+            }
+        }
+        if (head && cnt) {
+            unsigned long long* a = ps.jobs[j].accum + 4 * size_t(pix);
+            atomicAdd(a + 0, v0);
+            atomicAdd(a + 1, v1);
+            atomicAdd(a + 2, v2);
+            atomicAdd(a + 3, cnt);
+        }
+        // Finished samples: once per wave when the wave's paths share a job (the
+        // usual case), else once per run, on this block's lane of the job's
+        // spread counter.
         const unsigned long long live = __ballot(counts);
         if (live) {
+            const size_t dl = size_t(blockIdx.x % kDoneLanes) * kDoneStride;
             const int leader = __ffsll(static_cast<long long>(live)) - 1;
             const uint32_t j0 = __shfl(j, leader);
             if (__ballot(counts && j != j0) == 0) {
-                if (__lane_id() == leader) atomicAdd(ps.jobs[j0].done, static_cast<unsigned long long>(__popcll(live)));
-            } else if (counts) {
-                atomicAdd(job.done, 1ull);
+                if (lane == uint32_t(leader))
+                    atomicAdd(ps.jobs[j0].done + dl, static_cast<unsigned long long>(__popcll(live)));
+            } else if (head && cnt) {
+                atomicAdd(ps.jobs[j].done + dl, cnt);
             }
         }
-        ps.freeq[free0 + i] = slot;
+    }
+}
+
+// Mean radiance per pixel (reference PixelBuffer::average, image/pixel_buffer.cpp:37-45).
+__global__ void kResolve(const unsigned long long* accum, uint64_t npx, float* mean)
+{
+    for (uint64_t p = gridStart(); p < npx; p += gridStride()) {
+        const ulonglong2* a = reinterpret_cast<const ulonglong2*>(accum + 4 * p);
+        const ulonglong2 x = a[0], y = a[1];
+        const double n = double(y.y);
+        mean[3 * p + 0] = y.y ? float(double(x.x) / kFixedScale / n) : 0.0f;
+        mean[3 * p + 1] = y.y ? float(double(x.y) / kFixedScale / n) : 0.0f;
+        mean[3 * p + 2] = y.y ? float(double(y.x) / kFixedScale / n) : 0.0f;
     }
 }
 
@@ -725,31 +825,43 @@ hipError_t bounce(const PathState& ps, uint32_t grid, hipStream_t s)
     hipLaunchKernelGGL(kPlanBins, dim3(1), dim3(1024), 0, s, ps, grid);
     hipLaunchKernelGGL((kScatterBins<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     hipLaunchKernelGGL((kShade<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
+    hipLaunchKernelGGL((kCountShade<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     hipLaunchKernelGGL(kPlanShade, dim3(1), dim3(1024), 0, s, ps, grid);
     hipLaunchKernelGGL((kScatterShade<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     hipLaunchKernelGGL((kShadow<BLOCK, STACK>), dim3(grid), dim3(BLOCK), 0, s, ps);
-    hipLaunchKernelGGL(kFinish, dim3(grid), dim3(BLOCK), 0, s, ps);
+    hipLaunchKernelGGL((kFinish<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     hipLaunchKernelGGL(kPost, dim3(1), dim3(1), 0, s, ps);
     return hipGetLastError();
+}
+
+template <int BLOCK, int STACK>
+hipError_t bouncePair(const PathState& ps, uint32_t grid, hipStream_t s)
+{
+    PathState swapped = ps;
+    swapped.active = ps.active_next;
+    swapped.active_next = ps.active;
+    const hipError_t e = bounce<BLOCK, STACK>(ps, grid, s);
+    if (e != hipSuccess) return e;
+    return bounce<BLOCK, STACK>(swapped, grid, s);
 }
 
 template <int BLOCK>
 hipError_t bounceB(const PathState& ps, const LaunchShape& sh, hipStream_t s)
 {
-    if (sh.stack <= 32) return bounce<BLOCK, 32>(ps, sh.grid, s);
-    if (sh.stack <= 64) return bounce<BLOCK, 64>(ps, sh.grid, s);
+    if (sh.stack <= 32) return bouncePair<BLOCK, 32>(ps, sh.grid, s);
+    if (sh.stack <= 64) return bouncePair<BLOCK, 64>(ps, sh.grid, s);
     return hipErrorInvalidValue;
 }
 
 } // namespace
 
-hipError_t launchBounce(const PathState& ps, const LaunchShape& sh, hipStream_t s)
+hipError_t launchBouncePair(const PathState& ps, const LaunchShape& sh, hipStream_t s)
 {
     switch (sh.block) {
     case 64: return bounceB<64>(ps, sh, s);
     case 128: return bounceB<128>(ps, sh, s);
     case 256:
-        if (sh.stack <= 32) return bounce<256, 32>(ps, sh.grid, s);
+        if (sh.stack <= 32) return bouncePair<256, 32>(ps, sh.grid, s);
         return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
     }
@@ -758,6 +870,15 @@ hipError_t launchBounce(const PathState& ps, const LaunchShape& sh, hipStream_t 
 hipError_t launchReset(const PathState& ps, unsigned long long total, uint32_t n_jobs, hipStream_t s)
 {
     hipLaunchKernelGGL(kReset, dim3(1024), dim3(256), 0, s, ps, total, n_jobs);
+    return hipGetLastError();
+}
+
+hipError_t launchResolve(const unsigned long long* accum, uint64_t npx, float* mean, hipStream_t s)
+{
+    if (npx == 0) return hipSuccess;
+    const uint64_t want = (npx + 255) / 256;
+    const uint64_t blocks = want < 4096 ? want : 4096;
+    hipLaunchKernelGGL(kResolve, dim3(uint32_t(blocks)), dim3(256), 0, s, accum, npx, mean);
     return hipGetLastError();
 }
 

@@ -18,12 +18,22 @@ struct alignas(16) DevJob {
     uint32_t width, height, spa, max_depth;
     uint32_t roulette_start;
     uint32_t cancelled;
+    uint32_t group; // samples of one pixel that are consecutive in work order
+    uint32_t pad;
     uint64_t seed;
     uint64_t work_begin; // first global work index of this job
     uint64_t work_count; // width * height * spa^2
     unsigned long long* accum; // 4 per pixel: r, g, b fixed point (2^-28), sample count
-    unsigned long long* done;  // samples finished (one counter)
+    unsigned long long* done;  // samples finished: the sum of kDoneLanes counters (see below)
 };
+
+// A job's finished-sample count is spread over kDoneLanes counters kDoneStride
+// entries (256 B) apart, block b adding to counter b mod kDoneLanes: atomics on
+// one address serialise at about 12 ns each on MI355X, which made a single
+// counter cost ~100 us per half a million finished paths.
+constexpr uint32_t kDoneLanes = 64;
+constexpr uint32_t kDoneStride = 32;
+constexpr size_t kDoneBytes = size_t(kDoneLanes) * kDoneStride * 8;
 
 // Counters shared by the kernels of one engine.
 struct alignas(16) Counters {
@@ -64,8 +74,9 @@ struct PathState {
     float4* sh_org; // xyz origin, w tmax
     float4* sh_dir; // xyz direction
     float4* sh_c;   // xyz contribution if unoccluded
-    uint32_t* active;
-    uint32_t* binned;
+    uint32_t* active;      // paths to extend, in the order they were generated
+    uint32_t* active_next; // the next bounce's active queue (the two swap every bounce)
+    uint32_t* binned;      // positions in `active`, grouped by the material hit
     uint32_t* shadowq;
     uint32_t* termq;
     uint32_t* freeq;
@@ -81,12 +92,17 @@ struct LaunchShape {
     uint32_t stack;      // traversal stack depth (max over the jobs' scenes)
 };
 
-// Enqueues one bounce: generate, extend, bin, shade, shadow, finish, with the
-// bookkeeping kernels between them. No host synchronisation; capturable.
-hipError_t launchBounce(const PathState& ps, const LaunchShape& shape, hipStream_t s);
+// Enqueues two bounces (generate, extend, bin, shade, shadow, finish, with the
+// bookkeeping kernels between them): the first reads `active` and writes
+// `active_next`, the second the other way round, so the queues are back in
+// place afterwards. No host synchronisation; capturable.
+hipError_t launchBouncePair(const PathState& ps, const LaunchShape& shape, hipStream_t s);
 // Initialises counters and the free list for a fresh engine.
 hipError_t launchReset(const PathState& ps, unsigned long long work_total, uint32_t n_jobs, hipStream_t s);
 // Adds newly admitted work to the total (serving).
 hipError_t launchAddWork(const PathState& ps, unsigned long long work_total, uint32_t n_jobs, hipStream_t s);
+// Per-pixel mean radiance of a job: sums divided by the sample count, into
+// mean[npx * 3] (zero where no sample landed yet).
+hipError_t launchResolve(const unsigned long long* accum, uint64_t npx, float* mean, hipStream_t s);
 
 } // namespace ab

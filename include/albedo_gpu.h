@@ -289,6 +289,10 @@ ab_status ab_render_batch(ab_context* ctx, const ab_job_desc* jobs, uint32_t n_j
 ab_status ab_server_start(ab_context* ctx, uint64_t hbm_budget_bytes);
 ab_status ab_server_stop(ab_context* ctx);
 ab_status ab_job_submit(ab_context* ctx, const ab_job_desc* job, uint64_t* job_id);
+/* HBM bytes a job holds while admitted (accumulators, counter, mean image): the
+ * amount ab_job_submit charges against the budget. Validates the job like
+ * ab_job_submit. No counterpart in the reference; for callers that plan budgets. */
+ab_status ab_job_footprint(const ab_job_desc* job, uint64_t* bytes);
 /* Progressive snapshot: the mean over the samples finished so far. */
 ab_status ab_job_poll(ab_context* ctx, uint64_t job_id, ab_job_status* status, float* rgb_mean);
 ab_status ab_job_wait(ab_context* ctx, uint64_t job_id, ab_job_status* status, float* rgb_mean);

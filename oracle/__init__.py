@@ -53,6 +53,7 @@ def lib():
         L.or_scene_tlas_depth.argtypes = [_P]
         L.or_trace_closest.argtypes = [_P, _P, _U64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, _P, _P, _P]
         L.or_trace_any.argtypes = [_P, _P, _U64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]
+        L.or_trace_counts.argtypes = [_P, _P, _U64, ctypes.c_int, _P, _P]
         L.or_render.argtypes = [_P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, _P, _P]
         L.or_radiance.argtypes = [_P, _DP, _DP, _U64, ctypes.c_int, ctypes.c_int, _U64, _DP]
         L.or_pcg32.argtypes = [_U64, _U64, _U64, ctypes.c_uint32, _P]
@@ -113,6 +114,14 @@ class OracleScene:
                                threads or os.cpu_count() or 1, t.ctypes.data, obj.ctypes.data, prim.ctypes.data,
                                b1.ctypes.data, b2.ctypes.data)
         return t, obj, prim, b1, b2
+
+    def trace_counts(self, rays, threads=None):
+        """(box tests, primitive visits) of the FP64 ordered traversal, summed over rays."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        nodes, prims = ctypes.c_uint64(), ctypes.c_uint64()
+        lib().or_trace_counts(self.h, rays.ctypes.data, len(rays), threads or os.cpu_count() or 1,
+                              ctypes.byref(nodes), ctypes.byref(prims))
+        return nodes.value, prims.value
 
     def trace_any(self, rays, fp32=True, brute=False, threads=None):
         rays = np.ascontiguousarray(rays, dtype=np.float32)

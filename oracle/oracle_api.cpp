@@ -655,6 +655,27 @@ void or_trace_closest(const or_scene* s, const ab_ray* rays, uint64_t n, int mod
     });
 }
 
+// FP64 closest hits of n rays, counting the ordered traversal's work: box tests
+// (TLAS and BLAS nodes) and primitive visits (instances, spheres, triangles).
+void or_trace_counts(const or_scene* s, const ab_ray* rays, uint64_t n, int threads, uint64_t* nodes,
+                     uint64_t* prims)
+{
+    std::vector<uint64_t> cn(n), cp(n);
+    parallelFor(n, threads, [&](uint64_t i) {
+        tl_counts = TraversalCounts{};
+        (void)s->s->closest(rays[i], false, false);
+        cn[i] = tl_counts.nodes;
+        cp[i] = tl_counts.prims;
+    });
+    uint64_t a = 0, b = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        a += cn[i];
+        b += cp[i];
+    }
+    *nodes = a;
+    *prims = b;
+}
+
 void or_trace_any(const or_scene* s, const ab_ray* rays, uint64_t n, int mode, int brute, int threads,
                   uint8_t* occluded)
 {

@@ -19,6 +19,14 @@
 
 namespace oracle {
 
+// Work counters of the ordered traversal on the calling thread: box tests and
+// primitive visits. Only used to report how many bytes a traversal touches.
+struct TraversalCounts {
+    uint64_t nodes{0};
+    uint64_t prims{0};
+};
+inline thread_local TraversalCounts tl_counts;
+
 // ------------------------------------------------------------------ vectors
 
 template <class R>
@@ -377,6 +385,7 @@ public:
         const double widen = 1 + 2 * (3 * eps / 2) / (1 - 3 * eps / 2);
         while (true) {
             const BvhNode& n = nodes[cur];
+            ++tl_counts.nodes;
             double lo, hi;
             range(lo, hi);
             const double bl[3] = {n.box.lo.x, n.box.lo.y, n.box.lo.z};
@@ -396,8 +405,10 @@ public:
                     cur = first;
                     continue;
                 }
-                for (uint32_t e = n.offset; e < n.offset + n.count; ++e)
+                for (uint32_t e = n.offset; e < n.offset + n.count; ++e) {
+                    ++tl_counts.prims;
                     if (visit(order[e])) return;
+                }
             }
             if (sp == 0) return;
             cur = stack[--sp];

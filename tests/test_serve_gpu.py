@@ -43,7 +43,10 @@ def test_served_jobs_equal_plain_renders(ctx, scenes):
 
 def test_budget_rejects_and_serialises(ctx, scenes):
     room, _ = scenes
-    one_job = 64 * 64 * 32 + 256  # footprint of a 64x64 job, rounded up
+    one_job = Server.footprint(room, cam(1.0), 64, 64, 4, 8)
+    # accumulators (32 B per pixel), counter, mean image (12 B per pixel)
+    assert one_job >= 64 * 64 * 44 + 8
+    assert Server.footprint(room, cam(1.0), 128, 128, 1, 4) > one_job
     with Server(ctx, one_job) as srv:
         with pytest.raises(AlbedoError, match="AB_REJECTED_BUDGET"):
             srv.submit(room, cam(1.0), 128, 128, 1, 4)
