@@ -69,6 +69,20 @@ class Context:
         self.check(self.lib.ab_get_launch_config(self.handle, ctypes.byref(c)))
         return c
 
+    def autotune(self, jobs, batched=True):
+        """Picks and sets the fastest launch configuration for `jobs` (tuples as
+        for render_batch). Returns (config dict, trial wall ms)."""
+        descs = (_abi.JobDesc * len(jobs))()
+        for i, j in enumerate(jobs):
+            descs[i] = Server.job_desc(*j)
+        c = _abi.LaunchConfig()
+        ms = ctypes.c_double()
+        self.check(self.lib.ab_autotune(self.handle, descs, len(jobs),
+                                        _abi.AB_TUNE_BATCH if batched else _abi.AB_TUNE_PLAIN, ctypes.byref(c),
+                                        ctypes.byref(ms)))
+        return {"block_size": c.block_size, "paths_in_flight": c.paths_in_flight, "grid_blocks": c.grid_blocks,
+                "use_graph": c.use_graph}, ms.value
+
 
 class GpuScene:
     """A scene resident in HBM (ab_scene_create)."""

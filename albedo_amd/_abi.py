@@ -26,6 +26,8 @@ STATUS_NAMES = {0: "AB_OK", 1: "AB_INVALID_ARGUMENT", 2: "AB_OUT_OF_MEMORY", 3: 
                 4: "AB_REJECTED_BUDGET", 5: "AB_NOT_FOUND", 6: "AB_CANCELLED", 7: "AB_PENDING",
                 8: "AB_PARSE_ERROR"}
 
+AB_TUNE_PLAIN, AB_TUNE_BATCH = 0, 1
+
 AB_MISS = 0xFFFFFFFF
 AB_NO_PRIM = 0xFFFFFFFF
 
@@ -139,6 +141,8 @@ PROTOTYPES = {
     "ab_get_launch_config": (_S, [_P, ctypes.POINTER(LaunchConfig)]),
     "ab_render_batch": (_S, [_P, ctypes.POINTER(JobDesc), ctypes.c_uint32, ctypes.POINTER(_P),
                              ctypes.POINTER(RenderStats)]),
+    "ab_autotune": (_S, [_P, ctypes.POINTER(JobDesc), ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(LaunchConfig),
+                         ctypes.POINTER(ctypes.c_double)]),
     "ab_server_start": (_S, [_P, ctypes.c_uint64]),
     "ab_server_stop": (_S, [_P]),
     "ab_job_submit": (_S, [_P, ctypes.POINTER(JobDesc), ctypes.POINTER(ctypes.c_uint64)]),

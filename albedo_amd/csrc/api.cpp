@@ -90,7 +90,10 @@ ab_status ab_scene_create(ab_context* ctx, const ab_scene_desc* desc, ab_scene**
     } catch (const std::bad_alloc&) {
         return fail(AB_OUT_OF_MEMORY, "ab_scene_create: out of host memory building the BVH");
     }
-    const uint32_t stack = hs.tlas_depth + hs.max_blas_depth + 2;
+    // A traversal pushes one entry per interior node on its current root-to-leaf
+    // path, so it needs at most (levels - 1) entries per tree; a BLAS traversal
+    // starts on top of the TLAS entries pending at the instance's leaf.
+    const uint32_t stack = (hs.tlas_depth ? hs.tlas_depth - 1 : 0) + (hs.max_blas_depth ? hs.max_blas_depth - 1 : 0);
     if (stack > kMaxStackDepth)
         return fail(AB_INVALID_ARGUMENT, "ab_scene_create: BVH too deep for the traversal stack");
 

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -427,6 +428,84 @@ ab_status ab_render_batch(ab_context* ctx, const ab_job_desc* jobs, uint32_t n_j
         cfg = ctx->config;
     }
     return contextEngine(ctx).renderBatch(specs, outs, stats, cfg);
+}
+
+ab_status ab_autotune(ab_context* ctx, const ab_job_desc* jobs, uint32_t n_jobs, uint32_t mode,
+                      ab_launch_config* best, double* best_ms)
+{
+    if (!ctx || !jobs || n_jobs == 0) return fail(AB_INVALID_ARGUMENT, "ab_autotune: no jobs");
+    if (mode != AB_TUNE_PLAIN && mode != AB_TUNE_BATCH)
+        return fail(AB_INVALID_ARGUMENT, "ab_autotune: mode must be AB_TUNE_PLAIN or AB_TUNE_BATCH");
+    uint64_t pixels = 0;
+    for (uint32_t i = 0; i < n_jobs; ++i) {
+        if (!jobs[i].scene || jobs[i].scene->ctx != ctx)
+            return fail(AB_INVALID_ARGUMENT, "ab_autotune: job without a scene of this context");
+        pixels += uint64_t(jobs[i].params.width) * jobs[i].params.height;
+    }
+    // Trial size: the jobs' own resolutions, samples per axis capped so the
+    // whole trial is at most 2^24 camera samples.
+    uint32_t cap = 1;
+    while (uint64_t(cap + 1) * (cap + 1) * pixels <= (uint64_t(1) << 24)) ++cap;
+    std::vector<JobSpec> specs;
+    std::vector<std::vector<float>> images(n_jobs);
+    std::vector<float*> outs;
+    for (uint32_t i = 0; i < n_jobs; ++i) {
+        JobSpec s{jobs[i].scene, jobs[i].camera, jobs[i].params};
+        s.params.samples_per_axis = std::min(std::max<uint32_t>(s.params.samples_per_axis, 1), cap);
+        DevJob check{};
+        const ab_status st = prepareJob(s, check);
+        if (st != AB_OK) return st;
+        specs.push_back(s);
+        images[i].resize(size_t(s.params.width) * s.params.height * 3);
+        outs.push_back(images[i].data());
+    }
+    std::lock_guard<std::mutex> lock(ctx->render_mu);
+    ab_launch_config base;
+    {
+        std::lock_guard<std::mutex> l2(ctx->mu);
+        base = ctx->config;
+    }
+    Engine& engine = contextEngine(ctx);
+    auto trial = [&](const ab_launch_config& cfg, double& ms) -> ab_status {
+        const auto t0 = std::chrono::steady_clock::now();
+        if (mode == AB_TUNE_BATCH) {
+            const ab_status st = engine.renderBatch(specs, outs, nullptr, cfg);
+            if (st != AB_OK) return st;
+        } else {
+            for (size_t i = 0; i < specs.size(); ++i) {
+                const ab_status st = engine.renderBatch({specs[i]}, {outs[i]}, nullptr, cfg);
+                if (st != AB_OK) return st;
+            }
+        }
+        ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return AB_OK;
+    };
+    const uint32_t blocks[] = {128, 256};
+    const uint32_t paths[] = {1u << 20, 1u << 21, 1u << 22};
+    const uint32_t grids[] = {0, 4096};
+    ab_launch_config winner = base;
+    double winner_ms = 1e300;
+    for (uint32_t b : blocks)
+        for (uint32_t p : paths)
+            for (uint32_t g : grids) {
+                const ab_launch_config cfg{b, p, g, base.use_graph};
+                // The first run of a shape allocates and captures; time the second.
+                double ms = 0;
+                ab_status st = trial(cfg, ms);
+                if (st == AB_OK) st = trial(cfg, ms);
+                if (st != AB_OK) return st;
+                if (ms < winner_ms) {
+                    winner_ms = ms;
+                    winner = cfg;
+                }
+            }
+    {
+        std::lock_guard<std::mutex> l2(ctx->mu);
+        ctx->config = winner;
+    }
+    if (best) *best = winner;
+    if (best_ms) *best_ms = winner_ms;
+    return AB_OK;
 }
 
 ab_status ab_set_launch_config(ab_context* ctx, const ab_launch_config* c)

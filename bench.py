@@ -101,23 +101,27 @@ def batch_leg(ctx, n_jobs=16):
     for scene, (w, h), spa, depth, seed in mix:
         jobs.append((ctx.scene(scene), CameraSettings(c.look_from, c.look_at, (0, 1, 0), 38.0, w / h), w, h, spa,
                      depth, 3, seed))
-    albedo_amd.render_batch(ctx, jobs[:2])  # warm-up
+    base = ctx.launch_config()
+    # Each form runs with the launch configuration the autotuner picks for it.
+    cfg_plain, _ = ctx.autotune(jobs, batched=False)
     t0 = time.perf_counter()
     rays_plain = 0
     for j in jobs:
         _, st = j[0].render(*j[1:], stats=True)
         rays_plain += st["camera_rays"] + st["extension_rays"] + st["shadow_rays"]
     plain = time.perf_counter() - t0
+    cfg_batch, _ = ctx.autotune(jobs, batched=True)
     t0 = time.perf_counter()
     _, st = albedo_amd.render_batch(ctx, jobs)
     batched = time.perf_counter() - t0
+    ctx.set_launch_config(base.block_size, base.paths_in_flight, base.grid_blocks, base.use_graph)
     rays_b = st["camera_rays"] + st["extension_rays"] + st["shadow_rays"]
     samples = sum(j[2] * j[3] * j[4] ** 2 for j in jobs)
     return {"workload": "%d mixed jobs, 256x144..960x540, 4..64 spp, 5..200 random spheres" % n_jobs,
             "plain_s": round(plain, 3), "batched_s": round(batched, 3),
             "plain_mrays_s": round(rays_plain / plain / 1e6, 1), "batched_mrays_s": round(rays_b / batched / 1e6, 1),
             "batched_mpixel_spp_s": round(samples / batched / 1e6, 1),
-            "speedup": round(plain / batched, 3)}
+            "speedup": round(plain / batched, 3), "tuned_plain": cfg_plain, "tuned_batched": cfg_batch}
 
 
 def serve_leg(ctx, n_jobs=24, rate_hz=40.0):

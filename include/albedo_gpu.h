@@ -253,7 +253,7 @@ ab_status ab_render(ab_scene* scene, const ab_camera* camera, const ab_render_pa
 
 /* Launch-shape knobs of the wavefront engine. Results never depend on them. */
 typedef struct ab_launch_config {
-    uint32_t block_size;     /* threads per block for the queue kernels (64..1024) */
+    uint32_t block_size;     /* threads per block for the queue kernels: 64, 128 or 256 */
     uint32_t paths_in_flight;/* path slots resident at once */
     uint32_t grid_blocks;    /* blocks per launch of the grid-stride kernels, 0 = auto */
     uint32_t use_graph;      /* capture the per-bounce kernel chain as a hipGraph */
@@ -283,6 +283,19 @@ typedef struct ab_job_status {
  * Images go to rgb_means[i] (width*height*3 floats each). */
 ab_status ab_render_batch(ab_context* ctx, const ab_job_desc* jobs, uint32_t n_jobs,
                           float* const* rgb_means, ab_render_stats* stats);
+
+/* Autotuner: times trial renders of `jobs` under candidate launch
+ * configurations (block size, paths in flight, grid blocks) and sets the
+ * context's configuration to the fastest. Trials keep the jobs' scenes, cameras
+ * and resolutions and lower samples_per_axis so one trial takes at most 2^24
+ * camera samples. AB_TUNE_BATCH times one ab_render_batch call over all jobs,
+ * AB_TUNE_PLAIN one ab_render call per job. `best` and `best_ms` (wall time of
+ * the winning trial) may be NULL. Images never depend on the configuration, so
+ * tuning changes only speed. No counterpart in the reference. */
+#define AB_TUNE_PLAIN 0u
+#define AB_TUNE_BATCH 1u
+ab_status ab_autotune(ab_context* ctx, const ab_job_desc* jobs, uint32_t n_jobs, uint32_t mode,
+                      ab_launch_config* best, double* best_ms);
 
 /* Serving loop: a host thread admits submitted jobs as path slots and the HBM
  * budget allow, renders them together, and publishes progressive results. */

@@ -83,6 +83,27 @@ def test_launch_configuration_does_not_change_the_image(ctx, room):
                           images[0].view(np.uint32))
 
 
+def test_autotuner_picks_a_valid_configuration_and_keeps_images(ctx, room):
+    s, g, o = room
+    base = ctx.launch_config()
+    cam = CameraSettings(W.ROOM_CAMERA.look_from, W.ROOM_CAMERA.look_at, (0, 1, 0), 38.0, 48 / 27)
+    before = g.render(cam, 48, 27, 4, max_depth=8, roulette_start=3, seed=5)
+    jobs = [(g, cam, 48, 27, 4, 8, 3, 5), (g, cam, 32, 18, 2, 8, 3, 6)]
+    try:
+        for batched in (False, True):
+            cfg, ms = ctx.autotune(jobs, batched=batched)
+            assert cfg["block_size"] in (128, 256) and cfg["paths_in_flight"] >= 1 << 20 and ms > 0
+            now = ctx.launch_config()
+            assert (now.block_size, now.paths_in_flight, now.grid_blocks) == (
+                cfg["block_size"], cfg["paths_in_flight"], cfg["grid_blocks"])
+            after = g.render(cam, 48, 27, 4, max_depth=8, roulette_start=3, seed=5)
+            assert np.array_equal(after.view(np.uint32), before.view(np.uint32))
+    finally:
+        ctx.set_launch_config(base.block_size, base.paths_in_flight, base.grid_blocks, base.use_graph)
+    with pytest.raises(AlbedoError, match="AB_INVALID_ARGUMENT"):
+        ctx.autotune([])
+
+
 def batch(ctx, jobs):
     descs = (_abi.JobDesc * len(jobs))()
     outs = []
