@@ -35,6 +35,17 @@ uint32_t gridFor(uint32_t configured, uint32_t capacity, uint32_t block)
     return std::max<uint32_t>(64, std::min<uint32_t>((capacity + block - 1) / block, 2048));
 }
 
+// Traversal stack class of the bounce kernels for a scene stack depth.
+uint32_t stackClass(uint32_t depth) { return depth <= kShortStack ? kShortStack : (depth <= 32 ? 32u : 64u); }
+
+LaunchShape shapeFor(const ab_launch_config& cfg, uint32_t capacity, uint32_t stack)
+{
+    LaunchShape s{cfg.block_size, cfg.grid_blocks, stackClass(stack)};
+    if (s.block == 256 && s.stack > 32) s.block = 128; // 256 x 64 stack entries exceed the LDS budget
+    s.grid = gridFor(s.grid, capacity, s.block);
+    return s;
+}
+
 #define HIP_TRY(expr, what)                                   \
     do {                                                      \
         const hipError_t e_ = (expr);                         \
@@ -187,24 +198,42 @@ ab_status Engine::readCounters(Counters& out)
     return AB_OK;
 }
 
-ab_status Engine::runUntilDone(const LaunchShape& shape, bool use_graph, Counters& c)
+ab_status Engine::capture(const LaunchShape& shape)
 {
-    use_graph_ = use_graph;
+    if (!use_graph_) return AB_OK;
+    // The graph bakes in the path-state pointers, which change only when
+    // reserve() reallocates (and then drops the graph).
+    if (graph_ && shape.block == graph_shape_.block && shape.grid == graph_shape_.grid &&
+        shape.stack == graph_shape_.stack)
+        return AB_OK;
     if (graph_) {
         (void)hipGraphExecDestroy(graph_);
         graph_ = nullptr;
     }
-    if (use_graph) {
-        hipGraph_t g = nullptr;
-        HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-        const hipError_t le = launchBouncePair(ps_, shape, stream_);
-        const hipError_t ce = hipStreamEndCapture(stream_, &g);
-        if (le != hipSuccess) return hipFail(le, "bounce capture");
-        HIP_TRY(ce, "hipStreamEndCapture");
-        const hipError_t ie = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        HIP_TRY(ie, "hipGraphInstantiate");
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    const hipError_t le = launchBouncePair(ps_, shape, stream_);
+    const hipError_t ce = hipStreamEndCapture(stream_, &g);
+    if (le != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return hipFail(le, "bounce capture");
     }
+    HIP_TRY(ce, "hipStreamEndCapture");
+    const hipError_t ie = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+        graph_ = nullptr;
+        return hipFail(ie, "hipGraphInstantiate");
+    }
+    graph_shape_ = shape;
+    return AB_OK;
+}
+
+ab_status Engine::runUntilDone(const LaunchShape& shape, bool use_graph, Counters& c)
+{
+    use_graph_ = use_graph;
+    const ab_status cst = capture(shape);
+    if (cst != AB_OK) return cst;
     // Bounces between checks grow geometrically: the queue drains in a tail of
     // short bounces, and each check costs a host round trip.
     // Every path ends within max_depth + 1 bounces of its generation, and each
@@ -264,9 +293,7 @@ ab_status Engine::renderBatch(const std::vector<JobSpec>& specs, const std::vect
     work_total_ = total;
     HIP_TRY(hipMemcpyAsync(d_jobs_, jobs_.data(), sizeof(DevJob) * jobs_.size(), hipMemcpyHostToDevice, stream_),
             "hipMemcpy(jobs)");
-    LaunchShape shape{cfg.block_size, cfg.grid_blocks, stack <= 32 ? 32u : 64u};
-    shape.grid = gridFor(shape.grid, capacity, shape.block);
-    if (shape.block == 256 && shape.stack > 32) shape.block = 128;
+    const LaunchShape shape = shapeFor(cfg, capacity, stack);
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0), "hipEventCreate");
     HIP_TRY(hipEventCreate(&e1), "hipEventCreate");
@@ -312,26 +339,16 @@ ab_status Engine::beginStream(const ab_launch_config& cfg, uint32_t max_jobs)
     jobs_.clear();
     jobs_.reserve(max_jobs_);
     work_total_ = 0;
-    shape_ = LaunchShape{cfg.block_size == 256 ? 128u : cfg.block_size, cfg.grid_blocks, 64u};
-    shape_.grid = gridFor(shape_.grid, capacity, shape_.block);
-    HIP_TRY(launchReset(ps_, 0, 0, stream_), "reset launch");
+    stream_cfg_ = cfg;
     use_graph_ = cfg.use_graph != 0;
-    if (graph_) {
-        (void)hipGraphExecDestroy(graph_);
-        graph_ = nullptr;
-    }
-    if (use_graph_) {
-        hipGraph_t g = nullptr;
-        HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-        const hipError_t le = launchBouncePair(ps_, shape_, stream_);
-        const hipError_t ce = hipStreamEndCapture(stream_, &g);
-        if (le != hipSuccess) return hipFail(le, "bounce capture");
-        HIP_TRY(ce, "hipStreamEndCapture");
-        const hipError_t ie = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        HIP_TRY(ie, "hipGraphInstantiate");
-    }
-    return AB_OK;
+    HIP_TRY(launchReset(ps_, 0, 0, stream_), "reset launch");
+    return setStreamStack(kShortStack);
+}
+
+ab_status Engine::setStreamStack(uint32_t stack)
+{
+    shape_ = shapeFor(stream_cfg_, capacity_, stack);
+    return capture(shape_);
 }
 
 ab_status Engine::appendJob(const DevJob& prepared, unsigned long long* accum, unsigned long long* done,
@@ -340,6 +357,12 @@ ab_status Engine::appendJob(const DevJob& prepared, unsigned long long* accum, u
     if (tableFull()) return fail(AB_REJECTED_BUDGET, "job table full");
     if (prepared.scene.stack_depth > 64)
         return fail(AB_INVALID_ARGUMENT, "scene BVH too deep for the wavefront engine (stack > 64)");
+    // The bounce kernels follow the deepest scene in the table; queues do not
+    // depend on them, so they can change between bounces.
+    if (stackClass(prepared.scene.stack_depth) > shape_.stack) {
+        const ab_status st = setStreamStack(prepared.scene.stack_depth);
+        if (st != AB_OK) return st;
+    }
     DevJob dj = prepared;
     dj.work_begin = work_total_;
     dj.accum = accum;
@@ -378,7 +401,7 @@ ab_status Engine::resetTable()
     jobs_.clear();
     work_total_ = 0;
     HIP_TRY(launchReset(ps_, 0, 0, stream_), "reset launch");
-    return AB_OK;
+    return shape_.stack > kShortStack ? setStreamStack(kShortStack) : AB_OK;
 }
 
 Engine& contextEngine(ab_context* ctx)

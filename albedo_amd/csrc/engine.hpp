@@ -72,6 +72,10 @@ public:
 private:
     ab_status runUntilDone(const LaunchShape& shape, bool use_graph, Counters& final_counters);
     ab_status bouncePair(const LaunchShape& shape);
+    // (Re)captures the bounce-pair graph for `shape` unless it is current.
+    ab_status capture(const LaunchShape& shape);
+    // Serving: switches the bounce kernels to the stack class of `stack`.
+    ab_status setStreamStack(uint32_t stack);
     void freeAll();
 
     ab_context* ctx_;
@@ -84,8 +88,10 @@ private:
     Counters* h_counters_{nullptr}; // pinned
     std::vector<DevJob> jobs_;
     unsigned long long work_total_{0};
-    LaunchShape shape_{};
+    LaunchShape shape_{};          // serving: current bounce shape
+    ab_launch_config stream_cfg_{}; // serving: configuration the stream started with
     hipGraphExec_t graph_{nullptr};
+    LaunchShape graph_shape_{};     // shape graph_ was captured with
     bool use_graph_{false};
 };
 

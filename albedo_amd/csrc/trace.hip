@@ -22,7 +22,7 @@ __device__ __forceinline__ RayF loadRay(const float4* rays, uint64_t i)
 }
 
 template <int BLOCK, int STACK>
-__global__ __launch_bounds__(BLOCK) void kClosest(DevScene sc, const float4* __restrict__ rays,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(traversalWaves<STACK>(), 8))) void kClosest(DevScene sc, const float4* __restrict__ rays,
                                                   uint4* __restrict__ hits, uint64_t n)
 {
     __shared__ int32_t stack[STACK * BLOCK];
@@ -37,7 +37,7 @@ __global__ __launch_bounds__(BLOCK) void kClosest(DevScene sc, const float4* __r
 }
 
 template <int BLOCK, int STACK>
-__global__ __launch_bounds__(BLOCK) void kAny(DevScene sc, const float4* __restrict__ rays,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(traversalWaves<STACK>(), 8))) void kAny(DevScene sc, const float4* __restrict__ rays,
                                               uint8_t* __restrict__ occluded, uint64_t n)
 {
     __shared__ int32_t stack[STACK * BLOCK];
@@ -72,6 +72,7 @@ hipError_t launchTraceClosest(const DevScene& sc, const ab_ray* rays, ab_hit* hi
 {
     if (n == 0) return hipSuccess;
     if (n > (uint64_t(1) << 31) * 64) return hipErrorInvalidValue;
+    if (sc.stack_depth <= kShortStack) return launchC<128, kShortStack>(sc, rays, hits, n, s);
     if (sc.stack_depth <= 32) return launchC<128, 32>(sc, rays, hits, n, s);
     if (sc.stack_depth <= 64) return launchC<128, 64>(sc, rays, hits, n, s);
     if (sc.stack_depth <= 128) return launchC<64, 128>(sc, rays, hits, n, s);
@@ -83,6 +84,7 @@ hipError_t launchTraceAny(const DevScene& sc, const ab_ray* rays, uint8_t* occ, 
 {
     if (n == 0) return hipSuccess;
     if (n > (uint64_t(1) << 31) * 64) return hipErrorInvalidValue;
+    if (sc.stack_depth <= kShortStack) return launchA<128, kShortStack>(sc, rays, occ, n, s);
     if (sc.stack_depth <= 32) return launchA<128, 32>(sc, rays, occ, n, s);
     if (sc.stack_depth <= 64) return launchA<128, 64>(sc, rays, occ, n, s);
     if (sc.stack_depth <= 128) return launchA<64, 128>(sc, rays, occ, n, s);

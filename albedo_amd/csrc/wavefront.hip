@@ -848,6 +848,7 @@ hipError_t bouncePair(const PathState& ps, uint32_t grid, hipStream_t s)
 template <int BLOCK>
 hipError_t bounceB(const PathState& ps, const LaunchShape& sh, hipStream_t s)
 {
+    if (sh.stack <= kShortStack) return bouncePair<BLOCK, kShortStack>(ps, sh.grid, s);
     if (sh.stack <= 32) return bouncePair<BLOCK, 32>(ps, sh.grid, s);
     if (sh.stack <= 64) return bouncePair<BLOCK, 64>(ps, sh.grid, s);
     return hipErrorInvalidValue;
@@ -861,6 +862,7 @@ hipError_t launchBouncePair(const PathState& ps, const LaunchShape& sh, hipStrea
     case 64: return bounceB<64>(ps, sh, s);
     case 128: return bounceB<128>(ps, sh, s);
     case 256:
+        if (sh.stack <= kShortStack) return bouncePair<256, kShortStack>(ps, sh.grid, s);
         if (sh.stack <= 32) return bouncePair<256, 32>(ps, sh.grid, s);
         return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;

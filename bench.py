@@ -158,6 +158,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--trace-only", action="store_true", help="closest-hit benchmark only")
+    ap.add_argument("--legs", default="pt,batch,serve",
+                    help="legs after the closest-hit benchmark: any of pt, batch, serve (comma separated)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy, the reference for counter-based traffic")
     args = ap.parse_args()
@@ -247,9 +249,12 @@ def main():
         out["cpu_baseline"] = cpu_baseline(scene, [sets["coherent"], sets["incoherent"]])
     del d_rays, d_hits
     torch.cuda.empty_cache()
-    if not args.trace_only:
+    legs = set() if args.trace_only else set(args.legs.split(","))
+    if "pt" in legs:
         out["path_tracing"] = path_tracing_leg(ctx, cpu=not args.no_cpu)
+    if "batch" in legs:
         out["batched"] = batch_leg(ctx)
+    if "serve" in legs:
         out["served"] = serve_leg(ctx)
     print(json.dumps(out))
 
