@@ -19,14 +19,9 @@ constexpr uint32_t kMaxStackDepth = 256;
 
 // Traversal kernels keep their stack in LDS, STACK entries of 4 B per lane. With
 // a 20-entry stack a wave needs 5 KiB, so LDS leaves room for 8 waves per SIMD
-// and registers become the limit: the ray-query kernels then ask the compiler
-// for at least 7 waves per SIMD (72 VGPRs), which measured 15-20 % faster on
-// MI355X than the unconstrained 88 VGPRs (5 waves). The path tracer's extend and
-// shadow kernels spill at that target and measured no faster, so they stay
-// unconstrained. Deeper stacks are LDS-bound anyway.
+// and registers become the limit (the ray-query kernels then ask the compiler
+// for 6-7 waves per SIMD); deeper stacks are LDS-bound.
 constexpr uint32_t kShortStack = 20;
-template <int STACK>
-constexpr int traversalWaves() { return STACK <= int(kShortStack) ? 7 : 1; }
 
 // Maps block k of a G-block launch so each of the 8 XCDs (which receive blocks
 // round-robin) works on one contiguous range of logical blocks, keeping nearby

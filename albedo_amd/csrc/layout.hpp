@@ -110,7 +110,8 @@ struct DevScene {
     int32_t tlas_root;
     uint32_t num_lights;
     int32_t sky_light;
-    uint32_t stack_depth; // deepest path any traversal can need
+    uint32_t stack_depth; // most traversal-stack entries any query can hold at once
+    uint32_t has_spheres; // any object is a sphere (else ray queries skip the FP64 test)
 };
 
 } // namespace ab

@@ -1,6 +1,13 @@
-// Ray-query kernels: one lane per ray, LDS traversal stack, XCD-aware block
-// order. Reference: Scene::intersect / Scene::occluded (scene/scene.cpp:107-165)
-// over Bvh::nearestHit / anyHit (accel/bvh.hpp:211-267).
+// Ray-query kernels. Reference: Scene::intersect / Scene::occluded
+// (scene/scene.cpp:107-165) over Bvh::nearestHit / anyHit (accel/bvh.hpp:211-267).
+//
+// Each wave owns a contiguous slice of rays and keeps its 64 lanes busy: a lane
+// whose ray has finished takes the next ray of the slice (once at least kRefill
+// lanes are free), so a wave no longer waits for its longest ray. Incoherent
+// rays vary widely in traversal length; with one ray per lane the waves ran
+// with about 16 % of their lanes active. Results are written per ray and do not
+// depend on which lane or wave traced it. Slices are laid out so each XCD works
+// on a contiguous range of rays.
 
 #include "kernels.hpp"
 #include "traverse.hpp"
@@ -8,6 +15,11 @@
 namespace ab {
 
 namespace {
+
+constexpr uint32_t kBlock = 64;       // one wave per block: waves retire independently
+constexpr uint32_t kRefill = 16;      // free lanes that trigger a refill
+constexpr uint32_t kSteps = 16;       // traversal steps between refill checks
+constexpr uint32_t kTargetWaves = 32768;
 
 __device__ __forceinline__ uint32_t packBary(float b1, float b2)
 {
@@ -21,75 +33,93 @@ __device__ __forceinline__ RayF loadRay(const float4* rays, uint64_t i)
     return RayF{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 }
 
-template <int BLOCK, int STACK>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(traversalWaves<STACK>(), 8))) void kClosest(DevScene sc, const float4* __restrict__ rays,
-                                                  uint4* __restrict__ hits, uint64_t n)
+// Registers the compiler may use: with the FP64 sphere test compiled out the
+// query fits 72 VGPRs (7 waves per SIMD) with few spills; with it, 80 (6 waves).
+template <bool SPHERES>
+constexpr int queryWaves() { return SPHERES ? 6 : 7; }
+
+// One kernel for closest and any hit: `per_lane` rays per lane of each wave's slice.
+template <int STACK, bool ANY, bool SPHERES>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(queryWaves<SPHERES>(), 8))) void kQuery(
+    DevScene sc, const float4* __restrict__ rays, uint4* __restrict__ hits, uint8_t* __restrict__ occluded, uint64_t n,
+    uint32_t per_lane)
 {
-    __shared__ int32_t stack[STACK * BLOCK];
-    const uint64_t i = uint64_t(xcdRemap(blockIdx.x, gridDim.x)) * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const RayF r = loadRay(rays, i);
-    HitF h;
-    traceRay<BLOCK, false>(sc, r, h, LdsStack<BLOCK>{stack + threadIdx.x});
-    const bool miss = h.object == kMissId;
-    hits[i] = make_uint4(miss ? 0x7f800000u : __float_as_uint(h.t), h.object, h.prim,
-                         miss ? 0u : packBary(h.b1, h.b2));
+    __shared__ int32_t stack[STACK * kBlock];
+    const uint32_t lane = __lane_id();
+    const uint64_t slice = uint64_t(per_lane) * 64;
+    uint64_t next = uint64_t(xcdRemap(blockIdx.x, gridDim.x)) * slice;
+    const uint64_t end = next + slice < n ? next + slice : n;
+    const LdsStack<kBlock> stk{stack + threadIdx.x};
+    RayQuery<kBlock, ANY, SPHERES> q;
+    constexpr uint64_t kNone = ~0ull;
+    uint64_t id = kNone;
+    while (true) {
+        const bool idle = id == kNone;
+        const unsigned long long free_lanes = __ballot(idle);
+        if (next < end && (__popcll(free_lanes) >= kRefill || free_lanes == ~0ull)) {
+            const uint64_t mine = next + uint64_t(__popcll(free_lanes & ((1ull << lane) - 1ull)));
+            next += uint64_t(__popcll(free_lanes));
+            if (idle && mine < end) {
+                q.begin(sc, loadRay(rays, mine));
+                id = mine;
+            }
+        }
+        if (__ballot(id != kNone) == 0) break;
+        if (id != kNone) {
+            for (uint32_t k = 0; k < kSteps && !q.done; ++k) q.step(sc, stk);
+            if (q.done) {
+                if (ANY) {
+                    occluded[id] = q.best.object != kMissId ? 1 : 0;
+                } else {
+                    const bool miss = q.best.object == kMissId;
+                    hits[id] = make_uint4(miss ? 0x7f800000u : __float_as_uint(q.best.t), q.best.object, q.best.prim,
+                                          miss ? 0u : packBary(q.best.b1, q.best.b2));
+                }
+                id = kNone;
+            }
+        }
+    }
 }
 
-template <int BLOCK, int STACK>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(traversalWaves<STACK>(), 8))) void kAny(DevScene sc, const float4* __restrict__ rays,
-                                              uint8_t* __restrict__ occluded, uint64_t n)
+template <int STACK, bool ANY>
+hipError_t launchQ(const DevScene& sc, const ab_ray* rays, ab_hit* hits, uint8_t* occ, uint64_t n, hipStream_t s)
 {
-    __shared__ int32_t stack[STACK * BLOCK];
-    const uint64_t i = uint64_t(xcdRemap(blockIdx.x, gridDim.x)) * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const RayF r = loadRay(rays, i);
-    HitF h;
-    occluded[i] = traceRay<BLOCK, true>(sc, r, h, LdsStack<BLOCK>{stack + threadIdx.x}) ? 1 : 0;
-}
-
-template <int BLOCK, int STACK>
-hipError_t launchC(const DevScene& sc, const ab_ray* rays, ab_hit* hits, uint64_t n, hipStream_t s)
-{
-    const uint64_t blocks = (n + BLOCK - 1) / BLOCK;
-    hipLaunchKernelGGL((kClosest<BLOCK, STACK>), dim3(uint32_t(blocks)), dim3(BLOCK), 0, s, sc,
-                       reinterpret_cast<const float4*>(rays), reinterpret_cast<uint4*>(hits), n);
+    const uint64_t per_lane = (n + 64ull * kTargetWaves - 1) / (64ull * kTargetWaves);
+    const uint64_t waves = (n + 64 * per_lane - 1) / (64 * per_lane);
+    const float4* r = reinterpret_cast<const float4*>(rays);
+    uint4* h = reinterpret_cast<uint4*>(hits);
+    if (sc.has_spheres)
+        hipLaunchKernelGGL((kQuery<STACK, ANY, true>), dim3(uint32_t(waves)), dim3(kBlock), 0, s, sc, r, h, occ, n,
+                           uint32_t(per_lane));
+    else
+        hipLaunchKernelGGL((kQuery<STACK, ANY, false>), dim3(uint32_t(waves)), dim3(kBlock), 0, s, sc, r, h, occ, n,
+                           uint32_t(per_lane));
     return hipGetLastError();
 }
 
-template <int BLOCK, int STACK>
-hipError_t launchA(const DevScene& sc, const ab_ray* rays, uint8_t* occ, uint64_t n, hipStream_t s)
+template <bool ANY>
+hipError_t launchByStack(const DevScene& sc, const ab_ray* rays, ab_hit* hits, uint8_t* occ, uint64_t n, hipStream_t s)
 {
-    const uint64_t blocks = (n + BLOCK - 1) / BLOCK;
-    hipLaunchKernelGGL((kAny<BLOCK, STACK>), dim3(uint32_t(blocks)), dim3(BLOCK), 0, s, sc,
-                       reinterpret_cast<const float4*>(rays), occ, n);
-    return hipGetLastError();
+    if (n == 0) return hipSuccess;
+    if (n > (uint64_t(1) << 31) * 64) return hipErrorInvalidValue;
+    if (sc.stack_depth <= kShortStack) return launchQ<kShortStack, ANY>(sc, rays, hits, occ, n, s);
+    if (sc.stack_depth <= 32) return launchQ<32, ANY>(sc, rays, hits, occ, n, s);
+    if (sc.stack_depth <= 64) return launchQ<64, ANY>(sc, rays, hits, occ, n, s);
+    if (sc.stack_depth <= 128) return launchQ<128, ANY>(sc, rays, hits, occ, n, s);
+    if (sc.stack_depth <= kMaxStackDepth) return launchQ<256, ANY>(sc, rays, hits, occ, n, s);
+    return hipErrorInvalidValue;
 }
 
 } // namespace
 
 hipError_t launchTraceClosest(const DevScene& sc, const ab_ray* rays, ab_hit* hits, uint64_t n, hipStream_t s)
 {
-    if (n == 0) return hipSuccess;
-    if (n > (uint64_t(1) << 31) * 64) return hipErrorInvalidValue;
-    if (sc.stack_depth <= kShortStack) return launchC<128, kShortStack>(sc, rays, hits, n, s);
-    if (sc.stack_depth <= 32) return launchC<128, 32>(sc, rays, hits, n, s);
-    if (sc.stack_depth <= 64) return launchC<128, 64>(sc, rays, hits, n, s);
-    if (sc.stack_depth <= 128) return launchC<64, 128>(sc, rays, hits, n, s);
-    if (sc.stack_depth <= kMaxStackDepth) return launchC<64, 256>(sc, rays, hits, n, s);
-    return hipErrorInvalidValue;
+    return launchByStack<false>(sc, rays, hits, nullptr, n, s);
 }
 
 hipError_t launchTraceAny(const DevScene& sc, const ab_ray* rays, uint8_t* occ, uint64_t n, hipStream_t s)
 {
-    if (n == 0) return hipSuccess;
-    if (n > (uint64_t(1) << 31) * 64) return hipErrorInvalidValue;
-    if (sc.stack_depth <= kShortStack) return launchA<128, kShortStack>(sc, rays, occ, n, s);
-    if (sc.stack_depth <= 32) return launchA<128, 32>(sc, rays, occ, n, s);
-    if (sc.stack_depth <= 64) return launchA<128, 64>(sc, rays, occ, n, s);
-    if (sc.stack_depth <= 128) return launchA<64, 128>(sc, rays, occ, n, s);
-    if (sc.stack_depth <= kMaxStackDepth) return launchA<64, 256>(sc, rays, occ, n, s);
-    return hipErrorInvalidValue;
+    return launchByStack<true>(sc, rays, nullptr, occ, n, s);
 }
 
 } // namespace ab

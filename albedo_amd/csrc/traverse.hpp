@@ -378,4 +378,146 @@ __device__ __forceinline__ bool traceRay(const DevScene& sc, const RayF& r, HitF
     }
 }
 
+// The same two-level query as a resumable state machine: step() does one node
+// visit or one leaf, so a lane whose ray has finished can take a new ray while
+// the other lanes of its wave continue (see kernels that refill lanes). Visits
+// the same nodes and primitives with the same culling as traceRay, so results
+// are identical. A TLAS leaf is taken one object at a time; the leaf's other
+// objects wait on the stack as a shorter leaf reference, so the stack needs one
+// entry more than traceRay's. SPHERES = false compiles the FP64 sphere test out,
+// for scenes whose objects are all mesh instances.
+template <int BLOCK, bool ANY, bool SPHERES = true>
+struct RayQuery {
+    RayF r;          // world-space ray
+    BoxRay box;      // box-test form of the ray of the current level (world or instance space)
+    ShearRay sh;     // triangle-test form of the instance-space ray
+    HitF best;
+    int32_t ref;
+    int sp, sp_blas; // stack depth, and where the current instance's entries start
+    uint32_t obj;    // instance being traversed
+    bool in_blas, done;
+
+    __device__ __forceinline__ void begin(const DevScene& sc, const RayF& ray)
+    {
+        r = ray;
+        best.t = r.tmax;
+        best.object = kMissId;
+        best.prim = kMissId;
+        best.b1 = best.b2 = 0.0f;
+        box = makeBoxRay(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz);
+        ref = sc.tlas_root;
+        sp = sp_blas = 0;
+        in_blas = false;
+        done = sc.tlas_root == kNoChild;
+    }
+
+    // True when an any-hit query has found its hit.
+    __device__ __forceinline__ bool found() const { return ANY && best.object != kMissId; }
+
+    __device__ __forceinline__ void pop(LdsStack<BLOCK> stk)
+    {
+        if (in_blas && sp == sp_blas) {
+            // Back in the TLAS: the world-space box ray again.
+            in_blas = false;
+            box = makeBoxRay(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz);
+        }
+        if (sp == 0) {
+            done = true;
+            return;
+        }
+        ref = stk.get(--sp);
+    }
+
+    __device__ __forceinline__ void step(const DevScene& sc, LdsStack<BLOCK> stk)
+    {
+        if (ref >= 0) {
+            const DevNode n = loadNode(sc.nodes, ref);
+            bool h0, h1;
+            float t0, t1;
+            const float lim = ANY ? cullLimit(r.tmax) : cullLimit(best.object == kMissId ? r.tmax : best.t);
+            testChildren(n, box, r.tmin, lim, h0, h1, t0, t1);
+            if (h0 && h1) {
+                const bool swap = t1 < t0;
+                stk.put(sp++, swap ? n.child[0] : n.child[1]);
+                ref = swap ? n.child[1] : n.child[0];
+                return;
+            }
+            if (h0 || h1) {
+                ref = h0 ? n.child[0] : n.child[1];
+                return;
+            }
+            pop(stk);
+            return;
+        }
+        const uint32_t enc = uint32_t(~ref);
+        const uint32_t first = enc >> kLeafCountBits, count = enc & ((1u << kLeafCountBits) - 1);
+        if (in_blas) {
+            for (uint32_t i = 0; i < count; ++i) {
+                const float4* tp = reinterpret_cast<const float4*>(sc.tris + first + i);
+                const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
+                float t, b1, b2;
+                if (triHit(sh, v0, v1, v2, r.tmin, r.tmax, t, b1, b2)) {
+                    const uint32_t prim = __float_as_uint(v0.w);
+                    if (ANY || betterHit(t, obj, prim, best)) {
+                        best.t = t;
+                        best.object = obj;
+                        best.prim = prim;
+                        best.b1 = b1;
+                        best.b2 = b2;
+                    }
+                }
+            }
+            if (found()) {
+                done = true;
+                return;
+            }
+            pop(stk);
+            return;
+        }
+        if (count == 0) {
+            pop(stk);
+            return;
+        }
+        const uint32_t o = sc.leafobj[first];
+        if (count > 1) stk.put(sp++, encodeLeaf(first + 1, count - 1));
+        const DevObject& d = sc.objects[o];
+        if (SPHERES && d.kind == kObjSphere) {
+            double t;
+            if (sphereHit(sc.spheres + 4 * size_t(o), r, t)) {
+                const float tf = float(t);
+                if (ANY || betterHit(tf, o, kMissId, best)) {
+                    best.t = tf;
+                    best.object = o;
+                    best.prim = kMissId;
+                    best.b1 = 0.0f;
+                    best.b2 = 0.0f;
+                }
+            }
+            if (found()) {
+                done = true;
+                return;
+            }
+            pop(stk);
+            return;
+        }
+        float m[12];
+        const float4* mp = reinterpret_cast<const float4*>(d.to_object);
+        const float4 m0 = mp[0], m1 = mp[1], m2 = mp[2];
+        m[0] = m0.x; m[1] = m0.y; m[2] = m0.z; m[3] = m0.w;
+        m[4] = m1.x; m[5] = m1.y; m[6] = m1.z; m[7] = m1.w;
+        m[8] = m2.x; m[9] = m2.y; m[10] = m2.z; m[11] = m2.w;
+        const RayF ro = toObject(m, r);
+        sh = makeShear(ro.ox, ro.oy, ro.oz, ro.dx, ro.dy, ro.dz);
+        if (!sh.valid) {
+            pop(stk);
+            return;
+        }
+        box = makeBoxRay(ro.ox, ro.oy, ro.oz, ro.dx, ro.dy, ro.dz);
+        obj = o;
+        in_blas = true;
+        sp_blas = sp;
+        ref = d.blas_root;
+    }
+};
+
 } // namespace ab
