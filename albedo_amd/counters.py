@@ -19,7 +19,7 @@ import os
 import sys
 
 COPY_BYTES = 256 << 20
-KERNEL = "kClosest"
+KERNEL = "kQuery"
 RAYS = 1 << 24
 ALGORITHMIC_PER_RAY = 48
 
@@ -73,7 +73,7 @@ def main(argv):
     with open(out_json, "w") as fh:
         json.dump(s, fh, indent=1)
     with open(out_md, "w") as fh:
-        fh.write("# HBM traffic of kClosest (trefoil tube, 2^24 rays per launch)\n\n")
+        fh.write("# HBM traffic of the closest-hit kernel kQuery (trefoil tube, 2^24 rays per launch)\n\n")
         fh.write("Commands (repository root, ROCm 7.2 rocprofv3, one MI355X):\n\n")
         fh.write("    rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_fetch -o run -- "
                  "python3 bench.py --trace-only --copy-probe --steps 2 --warmup 1 --no-cpu\n")

@@ -56,7 +56,8 @@ def main(argv):
             continue
         act = c.get("SQ_ACTIVE_INST_VALU", 0)
         thr = c.get("SQ_THREAD_CYCLES_VALU", 0)
-        gui = c.get("GRBM_GUI_ACTIVE", 0)
+        # The CSV sums GRBM_GUI_ACTIVE over the 8 XCDs; the derived metrics use one XCD's count.
+        gui = c.get("GRBM_GUI_ACTIVE", 0) / 8
         wav = c.get("SQ_WAVE_CYCLES", 0)
         h = l2.get(k, {}).get("TCC_HIT_sum", 0)
         m = l2.get(k, {}).get("TCC_MISS_sum", 0)

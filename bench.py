@@ -240,7 +240,7 @@ def main():
         "per_set_mrays_s": {k: round(n_per / (v * 1e-3) / 1e6, 1) for k, v in kernel_ms.items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                     "kernel": "kClosest", "avg_launch_ms": round(avg_launch_ms, 4)},
+                     "kernel": "kQuery (closest hit)", "avg_launch_ms": round(avg_launch_ms, 4)},
     }
     hits = d_hits["incoherent"][:4096].cpu().numpy().view(np.uint32)
     if not np.any(hits[:, 1] != 0xFFFFFFFF):
