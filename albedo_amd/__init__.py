@@ -149,10 +149,12 @@ class GpuScene:
     # -------------------------------------------------------------- rendering
 
     def render(self, camera: CameraSettings, width, height, samples_per_axis, max_depth=64, roulette_start=3,
-               seed=0, stats=False):
-        """Mean radiance per pixel, float32 [height, width, 3]."""
+               seed=0, stats=False, estimator=_abi.ESTIMATOR_PATH):
+        """Mean radiance per pixel, float32 [height, width, 3]. estimator: one of
+        _abi.ESTIMATOR_PATH / ESTIMATOR_SCATTER_ONLY (max_depth = scatter limit) /
+        ESTIMATOR_NORMALS."""
         from .workloads import render_params
-        p = render_params(width, height, samples_per_axis, max_depth, roulette_start, seed)
+        p = render_params(width, height, samples_per_axis, max_depth, roulette_start, seed, estimator)
         cam = camera.to_abi()
         out = np.zeros((height, width, 3), dtype=np.float32)
         st = _abi.RenderStats()

@@ -27,6 +27,7 @@ STATUS_NAMES = {0: "AB_OK", 1: "AB_INVALID_ARGUMENT", 2: "AB_OUT_OF_MEMORY", 3: 
                 8: "AB_PARSE_ERROR"}
 
 AB_TUNE_PLAIN, AB_TUNE_BATCH = 0, 1
+ESTIMATOR_PATH, ESTIMATOR_SCATTER_ONLY, ESTIMATOR_NORMALS = 0, 1, 2
 
 AB_MISS = 0xFFFFFFFF
 AB_NO_PRIM = 0xFFFFFFFF
@@ -89,7 +90,7 @@ class Camera(ctypes.Structure):
 
 class RenderParams(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("samples_per_axis", ctypes.c_uint32),
-                ("max_depth", ctypes.c_uint32), ("roulette_start", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("max_depth", ctypes.c_uint32), ("roulette_start", ctypes.c_uint32), ("estimator", ctypes.c_uint32),
                 ("seed", ctypes.c_uint64)]
 
 

@@ -115,6 +115,9 @@ ab_status prepareJob(const JobSpec& spec, DevJob& dj)
     dj.max_depth = p.max_depth;
     dj.roulette_start = p.roulette_start;
     dj.group = std::min<uint32_t>(p.samples_per_axis * p.samples_per_axis, kPixelGroup);
+    if (p.estimator > AB_ESTIMATOR_NORMALS)
+        return fail(AB_INVALID_ARGUMENT, "unknown estimator " + std::to_string(p.estimator));
+    dj.estimator = p.estimator;
     dj.seed = p.seed;
     dj.work_count = uint64_t(p.width) * p.height * p.samples_per_axis * p.samples_per_axis;
     return AB_OK;

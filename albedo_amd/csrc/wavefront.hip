@@ -498,31 +498,44 @@ __device__ __forceinline__ uint32_t shadePath(const PathState& ps, uint32_t slot
         const int depth = __float_as_int(t4.w);
         const uint4 h = ps.hit[slot];
         const float nl = float(sc.num_lights);
+        // Estimators (reference estimator/radiance_estimator.hpp:36-38): the path
+        // estimator weighs emission by MIS and adds next-event estimation and
+        // roulette; the scatter-only one (scatter_only_estimator.cpp:10-48) only
+        // gathers the emission its BSDF samples run into; the normals view
+        // (normals_estimator.cpp:15-40) colours the first hit by its shading normal.
+        const uint32_t est = job.estimator;
+        const bool mis = est == AB_ESTIMATOR_PATH;
         bool alive = false, shadow = false;
         if (job.cancelled) {
             // terminated without gathering
         } else if (h.y == kMissId) {
-            if (sc.sky_light >= 0) {
+            if (est == AB_ESTIMATOR_NORMALS) {
+                // White straight down to (0.5, 0.7, 1) straight up, on the normalised direction.
+                const float up = 0.5f * (normalize(rd).y + 1.0f);
+                L = f3(1.0f, 1.0f, 1.0f) * (1.0f - up) + f3(0.5f, 0.7f, 1.0f) * up;
+            } else if (sc.sky_light >= 0) {
                 const DevLight& sky = sc.lights[sc.sky_light];
                 F3 e = skyRadiance(sky, rd);
-                if (!isBlack(e) && !arr_spec) e = e * powerHeuristic(arr_pdf, (1.0f / nl) * kUniformSpherePdf);
+                if (mis && !isBlack(e) && !arr_spec) e = e * powerHeuristic(arr_pdf, (1.0f / nl) * kUniformSpherePdf);
                 L = L + thr * e;
             }
         } else {
             const DevObject& obj = sc.objects[h.y];
             const DevMaterial* m = &sc.materials[obj.material];
-            const bool uv_needed = m->kind == 0u || m->kind == 3u;
+            const bool uv_needed = est != AB_ESTIMATOR_NORMALS && (m->kind == 0u || m->kind == 3u);
             const SurfaceHit s = surfaceAt(sc, h, ps.hit_b2[slot], ro, rd, uv_needed);
-            if (obj.light >= 0 && s.front) {
+            if (est == AB_ESTIMATOR_NORMALS) {
+                L = (s.ns + f3(1.0f, 1.0f, 1.0f)) * 0.5f;
+            } else if (obj.light >= 0 && s.front) {
                 const DevLight& l = sc.lights[obj.light];
                 if (l.kind == 1u) {
                     F3 e = f3(l.a[0], l.a[1], l.a[2]);
-                    if (!isBlack(e) && !arr_spec)
+                    if (mis && !isBlack(e) && !arr_spec)
                         e = e * powerHeuristic(arr_pdf, (1.0f / nl) * lightPdf(l, arr_o, rd));
                     L = L + thr * e;
                 }
             }
-            if (depth < int(job.max_depth)) {
+            if (est != AB_ESTIMATOR_NORMALS && depth < int(job.max_depth)) {
                 Rng rng;
                 rng.state = ps.rng[slot];
                 rng.inc = (mixBits(ps.pixel[slot]) << 1u) | 1u;
@@ -543,8 +556,9 @@ __device__ __forceinline__ uint32_t shadePath(const PathState& ps, uint32_t slot
                 }
                 const F3 wo = -rd;
                 // Next-event estimation (path_estimator.cpp:73-102): three draws always.
-                const float ul = rng.uniform(), lx = rng.uniform(), ly = rng.uniform();
-                if (sc.num_lights > 0) {
+                const float ul = mis ? rng.uniform() : 0.0f, lx = mis ? rng.uniform() : 0.0f,
+                            ly = mis ? rng.uniform() : 0.0f;
+                if (mis && sc.num_lights > 0) {
                     const uint32_t li = min(uint32_t(ul * nl), sc.num_lights - 1);
                     const DevLight& l = sc.lights[li];
                     LightSampleF ls;
@@ -576,7 +590,7 @@ __device__ __forceinline__ uint32_t shadePath(const PathState& ps, uint32_t slot
                     thr = thr * smp.weight;
                     if (!isBlack(thr)) {
                         alive = true;
-                        if (depth >= int(job.roulette_start)) {
+                        if (mis && depth >= int(job.roulette_start)) {
                             const float surv = fminf(0.95f, maxc(thr));
                             if (rng.uniform() >= surv) alive = false;
                             else thr = thr / surv;

@@ -18,8 +18,8 @@ struct alignas(16) DevJob {
     uint32_t width, height, spa, max_depth;
     uint32_t roulette_start;
     uint32_t cancelled;
-    uint32_t group; // samples of one pixel that are consecutive in work order
-    uint32_t pad;
+    uint32_t group;     // samples of one pixel that are consecutive in work order
+    uint32_t estimator; // AB_ESTIMATOR_*
     uint64_t seed;
     uint64_t work_begin; // first global work index of this job
     uint64_t work_count; // width * height * spa^2

@@ -620,11 +620,17 @@ private:
     CameraSettings s_;
 };
 
+// The estimators of reference estimator/radiance_estimator.hpp:36-38; each is a
+// mode of the device path tracer (ab_render_params::estimator).
+struct NormalsEstimator {};
+struct ScatterOnlyEstimator {
+    int max_scatter_count{32};
+};
 struct PathEstimator {
     int max_depth{64};
     int roulette_start{3};
 };
-using RadianceEstimator = std::variant<PathEstimator>;
+using RadianceEstimator = std::variant<NormalsEstimator, ScatterOnlyEstimator, PathEstimator>;
 
 // Per-pixel mean radiance (the reference keeps sums and counts; the device keeps
 // them as fixed-point sums and returns the means).
@@ -667,9 +673,20 @@ inline void renderImage(const RenderJob& job, PixelBuffer& image)
         throw std::invalid_argument("samples per axis must be positive, got " + std::to_string(job.samples_per_axis));
     if (job.thread_count <= 0)
         throw std::invalid_argument("thread count must be positive, got " + std::to_string(job.thread_count));
-    const PathEstimator& est = std::get<PathEstimator>(job.estimator);
-    if (est.max_depth < 0 || est.roulette_start < 0)
-        throw std::invalid_argument("path limits must not be negative");
+    ab_render_params p{};
+    if (const auto* path = std::get_if<PathEstimator>(&job.estimator)) {
+        if (path->max_depth < 0 || path->roulette_start < 0)
+            throw std::invalid_argument("path limits must not be negative");
+        p.estimator = AB_ESTIMATOR_PATH;
+        p.max_depth = uint32_t(path->max_depth);
+        p.roulette_start = uint32_t(path->roulette_start);
+    } else if (const auto* scatter = std::get_if<ScatterOnlyEstimator>(&job.estimator)) {
+        if (scatter->max_scatter_count < 0) throw std::invalid_argument("scatter count limit must not be negative");
+        p.estimator = AB_ESTIMATOR_SCATTER_ONLY;
+        p.max_depth = uint32_t(scatter->max_scatter_count);
+    } else {
+        p.estimator = AB_ESTIMATOR_NORMALS;
+    }
     const CameraSettings& cs = job.camera.get().settings();
     ab_camera cam{};
     const Real from[3] = {cs.look_from.x, cs.look_from.y, cs.look_from.z};
@@ -682,12 +699,9 @@ inline void renderImage(const RenderJob& job, PixelBuffer& image)
     }
     cam.vfov_degrees = cs.vertical_fov_degrees;
     cam.aspect = cs.aspect_ratio;
-    ab_render_params p{};
     p.width = uint32_t(image.width());
     p.height = uint32_t(image.height());
     p.samples_per_axis = uint32_t(job.samples_per_axis);
-    p.max_depth = uint32_t(est.max_depth);
-    p.roulette_start = uint32_t(est.roulette_start);
     p.seed = job.seed;
     detail::check(ab_render(job.scene.get().device(), &cam, &p, image.data(), nullptr));
 }

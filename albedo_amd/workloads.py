@@ -316,11 +316,13 @@ def job_mix(n_jobs, seed=3, resolutions=RESOLUTIONS, spas=SPAS, max_spheres=200,
     return jobs
 
 
-def render_params(width, height, samples_per_axis, max_depth, roulette_start=3, seed=0):
+def render_params(width, height, samples_per_axis, max_depth, roulette_start=3, seed=0,
+                  estimator=_abi.ESTIMATOR_PATH):
     p = _abi.RenderParams()
     p.width, p.height = width, height
     p.samples_per_axis = samples_per_axis
     p.max_depth = max_depth
     p.roulette_start = roulette_start
+    p.estimator = estimator
     p.seed = seed
     return p

@@ -176,16 +176,26 @@ typedef struct ab_camera {
     double aspect;
 } ab_camera;
 
-/* Render parameters (reference RenderJob + PathEstimator,
- * render/render_image.hpp:17-32, estimator/path_estimator.hpp:22-38).
- * Each pixel takes samples_per_axis^2 stratified jittered samples. */
+/* Render parameters (reference RenderJob + the RadianceEstimator alternatives,
+ * render/render_image.hpp:17-32, estimator/radiance_estimator.hpp:36-38).
+ * Each pixel takes samples_per_axis^2 stratified jittered samples.
+ *   AB_ESTIMATOR_PATH          PathEstimator{max_depth, roulette_start}
+ *                              (estimator/path_estimator.hpp:22-38)
+ *   AB_ESTIMATOR_SCATTER_ONLY  ScatterOnlyEstimator{max_scatter_count = max_depth}
+ *                              (estimator/scatter_only_estimator.hpp:19-26);
+ *                              roulette_start is unused
+ *   AB_ESTIMATOR_NORMALS       NormalsEstimator (estimator/normals_estimator.hpp:14-26);
+ *                              max_depth and roulette_start are unused */
+#define AB_ESTIMATOR_PATH 0u
+#define AB_ESTIMATOR_SCATTER_ONLY 1u
+#define AB_ESTIMATOR_NORMALS 2u
 typedef struct ab_render_params {
     uint32_t width;
     uint32_t height;
     uint32_t samples_per_axis;
     uint32_t max_depth;
     uint32_t roulette_start;
-    uint32_t reserved;
+    uint32_t estimator;      /* AB_ESTIMATOR_* */
     uint64_t seed;
 } ab_render_params;
 

@@ -554,6 +554,48 @@ inline Rgb radiance(const Scene& s, V3d ro, V3d rd, Generator& g, int max_depth,
     }
 }
 
+// reference estimator/scatter_only_estimator.cpp:10-48
+inline Rgb scatterOnly(const Scene& s, V3d ro, V3d rd, Generator& g, int max_count, RayCounts& rc)
+{
+    Rgb thr{1, 1, 1}, L{};
+    for (int count = 0;; ++count) {
+        if (count == 0) ++rc.camera;
+        else ++rc.extension;
+        const auto hit = s.intersect(ro, rd);
+        if (!hit) return L + thr * s.skyRadiance(rd);
+        L = L + thr * s.emitted(*hit);
+        if (count >= max_count) return L;
+        const double uc = g.next1D();
+        double ux, uy;
+        g.next2D(ux, uy);
+        const auto smp = s.scatteringAt(*hit).sample(-rd, uc, ux, uy);
+        if (!smp) return L;
+        thr = thr * smp->weight;
+        if (isBlack(thr)) return L;
+        const double side = dot(smp->direction, hit->normal) >= 0 ? 1.0 : -1.0;
+        ro = hit->point + hit->normal * (side * kSurfaceOffset);
+        rd = smp->direction;
+    }
+}
+
+// reference estimator/normals_estimator.cpp:15-40
+inline Rgb normalsView(const Scene& s, V3d ro, V3d rd, RayCounts& rc)
+{
+    ++rc.camera;
+    const auto hit = s.intersect(ro, rd);
+    if (hit) return {0.5 * (hit->shading.x + 1), 0.5 * (hit->shading.y + 1), 0.5 * (hit->shading.z + 1)};
+    const double h = 0.5 * (normalized(rd).y + 1);
+    return Rgb{1, 1, 1} * (1 - h) + Rgb{0.5, 0.7, 1.0} * h;
+}
+
+// The estimator an ab_render_params selects (reference radiance_estimator.hpp:36-38).
+inline Rgb estimate(const Scene& s, const ab_render_params& p, V3d ro, V3d rd, Generator& g, RayCounts& rc)
+{
+    if (p.estimator == AB_ESTIMATOR_NORMALS) return normalsView(s, ro, rd, rc);
+    if (p.estimator == AB_ESTIMATOR_SCATTER_ONLY) return scatterOnly(s, ro, rd, g, int(p.max_depth), rc);
+    return radiance(s, ro, rd, g, int(p.max_depth), int(p.roulette_start), rc);
+}
+
 struct Camera {
     V3d origin, forward, half_w, half_h;
     explicit Camera(const ab_camera& c)
@@ -707,7 +749,7 @@ void or_render(const or_scene* s, const ab_camera* cam, const ab_render_params* 
                 const double fx = (x + (sx + jx) / n) / double(W);
                 const double fy = (y + (sy + jy) / n) / double(H);
                 const V3d d = camera.direction(fx, fy);
-                sum = sum + radiance(*s->s, camera.origin, d, g, int(p->max_depth), int(p->roulette_start), rc);
+                sum = sum + estimate(*s->s, *p, camera.origin, d, g, rc);
             }
         const Rgb mean = sum / double(n * n);
         const uint64_t o = (uint64_t(y) * W + x) * 3;

@@ -49,6 +49,23 @@ int main(int argc, char** argv)
         }
     std::printf("mean %.6f\n", sum / (3.0 * width * height));
 
+    // The other estimators of the reference's RadianceEstimator variant.
+    PixelBuffer normals{width, height};
+    renderImage(RenderJob{camera, scene, 1, 7, NormalsEstimator{}, 1}, normals);
+    PixelBuffer scatter{width, height};
+    renderImage(RenderJob{camera, scene, 2, 7, ScatterOnlyEstimator{8}, 1}, scatter);
+    double nmin = 1, nmax = 0, ssum = 0;
+    for (int y = 0; y < height; ++y)
+        for (int x = 0; x < width; ++x) {
+            const Rgb c = normals.average({x, y});
+            nmin = std::min({nmin, double(c.r), double(c.g), double(c.b)});
+            nmax = std::max({nmax, double(c.r), double(c.g), double(c.b)});
+            const Rgb d = scatter.average({x, y});
+            ssum += d.r + d.g + d.b;
+        }
+    std::printf("normals in [%.3f, %.3f], scatter-only mean %.6f\n", nmin, nmax, ssum / (3.0 * width * height));
+    if (nmin < 0 || nmax > 1 || !(ssum > 0)) return 1;
+
     int errors_seen = 0;
     try {
         renderImage(RenderJob{camera, scene, 0, 7, PathEstimator{}, 1}, image);

@@ -338,3 +338,57 @@ def test_mirror_and_smooth_glass_are_specular():
     assert abs(refr[1][0] - 1 / 1.5 ** 2) < 1e-15
     # Snell: sin_t = sin_i / eta
     assert abs(math.hypot(refr[0][0], refr[0][1]) - 0.6 / 1.5) < 1e-12
+
+
+# ------------------------------------------------------ estimators (closed forms)
+
+def _one_sphere_scene(sky=None, albedo=0.6):
+    from albedo_amd.scene import ConstantTexture, DiffuseMaterial, Scene, SkyLight, Sphere
+    s = Scene()
+    s.add(Sphere((0.0, 0.0, 0.0), 1.0), DiffuseMaterial(ConstantTexture((albedo, albedo, albedo))))
+    if sky is not None:
+        s.set_sky(SkyLight.uniform(sky))
+    return oracle.OracleScene(s.to_desc())
+
+
+def _narrow_camera(look_at):
+    from albedo_amd import _abi
+    c = _abi.Camera()
+    c.look_from[:] = (0.0, 0.0, 5.0)
+    c.look_at[:] = look_at
+    c.up[:] = (0.0, 1.0, 0.0)
+    c.vfov_degrees = 0.001  # a pixel spans 4e-5 at the sphere
+    c.aspect = 1.0
+    return c
+
+
+def test_normals_view_closed_forms():
+    # Reference normals_estimator.cpp:15-40: a hit shows 0.5 (n + 1); a miss the
+    # white-to-blue gradient on the direction's height.
+    from albedo_amd import _abi
+    from albedo_amd import workloads as W
+    o = _one_sphere_scene()
+    p = W.render_params(1, 1, 2, 8, 3, 0, _abi.ESTIMATOR_NORMALS)
+    img, counts = o.render(_narrow_camera((0.0, 0.0, 0.0)), p, sample_mode=0)
+    assert np.allclose(img[0, 0], [0.5, 0.5, 1.0], atol=1e-4)
+    assert counts[1] == 0 and counts[2] == 0
+    img, _ = o.render(_narrow_camera((0.0, 0.0, 10.0)), p, sample_mode=0)  # looking away: a miss along +z
+    assert np.allclose(img[0, 0], [0.75, 0.85, 1.0], atol=1e-4)
+
+
+def test_scatter_only_convex_object_under_uniform_sky_is_exact():
+    # Reference scatter_only_estimator.cpp:10-48: a ray that hits a lone convex
+    # diffuse object scatters once (weight = albedo exactly, lambert_lobe.cpp) and
+    # then escapes, so every sample is albedo x sky radiance.
+    from albedo_amd import _abi
+    from albedo_amd import workloads as W
+    o = _one_sphere_scene(sky=(0.5, 0.5, 0.5), albedo=0.6)
+    p = W.render_params(1, 1, 8, 32, 3, 4, _abi.ESTIMATOR_SCATTER_ONLY)
+    img, counts = o.render(_narrow_camera((0.0, 0.0, 0.0)), p, sample_mode=0)
+    exact = float(np.float32(0.6)) * float(np.float32(0.5))  # the scene stores colours in FP32
+    assert np.allclose(img[0, 0], exact, rtol=1e-12)
+    assert counts[0] == 64 and counts[1] == 64 and counts[2] == 0
+    # A planted mistake: dropping the sky term at the escape would give black.
+    p.max_depth = 0  # scatter limit 0: the path ends at its first hit, gathering nothing
+    img, _ = o.render(_narrow_camera((0.0, 0.0, 0.0)), p, sample_mode=0)
+    assert np.allclose(img[0, 0], 0.0)
