@@ -178,10 +178,16 @@ class Server:
 
     QUEUED, RUNNING, DONE, CANCELLED, FAILED = 0, 1, 2, 3, 4
 
-    def __init__(self, ctx: Context, hbm_budget_bytes: int):
+    def __init__(self, ctx: Context, hbm_budget_bytes: int, policy=None):
         self.ctx = ctx
         ctx.check(ctx.lib.ab_server_start(ctx.handle, hbm_budget_bytes))
         self.shapes = {}
+        if policy is not None:
+            self.set_policy(policy)
+
+    def set_policy(self, policy):
+        """_abi.SCHED_FIFO, SCHED_SHARE or SCHED_SMALL_FIRST (the default)."""
+        self.ctx.check(self.ctx.lib.ab_server_set_policy(self.ctx.handle, policy))
 
     def stop(self):
         if self.ctx is not None:

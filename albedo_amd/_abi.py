@@ -28,6 +28,7 @@ STATUS_NAMES = {0: "AB_OK", 1: "AB_INVALID_ARGUMENT", 2: "AB_OUT_OF_MEMORY", 3: 
 
 AB_TUNE_PLAIN, AB_TUNE_BATCH = 0, 1
 ESTIMATOR_PATH, ESTIMATOR_SCATTER_ONLY, ESTIMATOR_NORMALS = 0, 1, 2
+SCHED_FIFO, SCHED_SHARE, SCHED_SMALL_FIRST = 0, 1, 2
 
 AB_MISS = 0xFFFFFFFF
 AB_NO_PRIM = 0xFFFFFFFF
@@ -146,6 +147,7 @@ PROTOTYPES = {
                          ctypes.POINTER(ctypes.c_double)]),
     "ab_server_start": (_S, [_P, ctypes.c_uint64]),
     "ab_server_stop": (_S, [_P]),
+    "ab_server_set_policy": (_S, [_P, ctypes.c_uint32]),
     "ab_job_submit": (_S, [_P, ctypes.POINTER(JobDesc), ctypes.POINTER(ctypes.c_uint64)]),
     "ab_job_footprint": (_S, [ctypes.POINTER(JobDesc), ctypes.POINTER(ctypes.c_uint64)]),
     "ab_job_poll": (_S, [_P, ctypes.c_uint64, ctypes.POINTER(JobStatus), _P]),

@@ -92,10 +92,11 @@ ab_status ab_scene_create(ab_context* ctx, const ab_scene_desc* desc, ab_scene**
     }
     // A traversal pushes one entry per interior node on its current root-to-leaf
     // path, so it needs at most (levels - 1) entries per tree; a BLAS traversal
-    // starts on top of the TLAS entries pending at the instance's leaf, plus one
-    // entry that the resumable query (RayQuery) keeps for the leaf's other objects.
-    const uint32_t stack =
-        (hs.tlas_depth ? hs.tlas_depth - 1 : 0) + (hs.max_blas_depth ? hs.max_blas_depth - 1 : 0) + 1;
+    // starts on top of the TLAS entries pending at the instance's leaf (traceRay),
+    // and the resumable query (RayQuery) keeps one more for the leaf's other objects.
+    const uint32_t path_stack =
+        (hs.tlas_depth ? hs.tlas_depth - 1 : 0) + (hs.max_blas_depth ? hs.max_blas_depth - 1 : 0);
+    const uint32_t stack = path_stack + 1;
     if (stack > kMaxStackDepth)
         return fail(AB_INVALID_ARGUMENT, "ab_scene_create: BVH too deep for the traversal stack");
 
@@ -151,6 +152,7 @@ ab_status ab_scene_create(ab_context* ctx, const ab_scene_desc* desc, ab_scene**
     s->dev.num_lights = uint32_t(hs.lights.size());
     s->dev.sky_light = hs.sky_light;
     s->dev.stack_depth = stack;
+    s->dev.path_stack_depth = path_stack;
     s->dev.has_spheres = 0;
     for (const DevObject& o : hs.objects) s->dev.has_spheres |= o.kind == kObjSphere ? 1u : 0u;
     s->tlas_depth = hs.tlas_depth;

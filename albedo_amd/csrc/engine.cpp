@@ -132,11 +132,15 @@ void Engine::freeAll()
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (arena_) (void)hipFree(arena_);
     if (h_counters_) (void)hipHostFree(h_counters_);
+    if (h_segs_) (void)hipHostFree(h_segs_);
     if (stream_) (void)hipStreamDestroy(stream_);
     arena_ = nullptr;
     h_counters_ = nullptr;
+    h_segs_ = nullptr;
+    d_segs_ = nullptr;
     stream_ = nullptr;
     capacity_ = max_jobs_ = 0;
+    seg_cap_ = seg_lo_ = seg_hi_ = seg_sent_ = 0;
 }
 
 ab_status Engine::reserve(uint32_t capacity, uint32_t max_jobs, hipStream_t)
@@ -147,6 +151,10 @@ ab_status Engine::reserve(uint32_t capacity, uint32_t max_jobs, hipStream_t)
     HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&h_counters_), sizeof(Counters), hipHostMallocDefault),
             "hipHostMalloc(counters)");
+    uint32_t seg_cap = 4096;
+    while (seg_cap < max_jobs) seg_cap *= 2;
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&h_segs_), sizeof(DevSegment) * seg_cap, hipHostMallocDefault),
+            "hipHostMalloc(segments)");
     const size_t C = capacity;
     struct Part {
         void** dst;
@@ -166,7 +174,8 @@ ab_status Engine::reserve(uint32_t capacity, uint32_t max_jobs, hipStream_t)
                     {reinterpret_cast<void**>(&p.block_count), size_t(kMaxGrid) * 8 * 4},
                     {reinterpret_cast<void**>(&p.block_off), size_t(kMaxGrid) * 8 * 4},
                     {reinterpret_cast<void**>(&p.ctr), sizeof(Counters)},
-                    {reinterpret_cast<void**>(&d_jobs_), sizeof(DevJob) * std::max<uint32_t>(max_jobs, 1)}};
+                    {reinterpret_cast<void**>(&d_jobs_), sizeof(DevJob) * std::max<uint32_t>(max_jobs, 1)},
+                    {reinterpret_cast<void**>(&d_segs_), sizeof(DevSegment) * seg_cap}};
     size_t total = 0;
     for (const Part& part : parts) total += align256(part.bytes);
     HIP_TRY(hipMalloc(&arena_, total), "hipMalloc(path state)");
@@ -177,6 +186,9 @@ ab_status Engine::reserve(uint32_t capacity, uint32_t max_jobs, hipStream_t)
     }
     p.capacity = capacity;
     p.jobs = d_jobs_;
+    p.segs = d_segs_;
+    p.seg_mask = seg_cap - 1;
+    seg_cap_ = seg_cap;
     capacity_ = capacity;
     max_jobs_ = std::max<uint32_t>(max_jobs, 1);
     return AB_OK;
@@ -273,9 +285,8 @@ ab_status Engine::renderBatch(const std::vector<JobSpec>& specs, const std::vect
     for (size_t i = 0; i < specs.size(); ++i) {
         const ab_status st = prepareJob(specs[i], jobs_[i]);
         if (st != AB_OK) return st;
-        jobs_[i].work_begin = total;
         total += jobs_[i].work_count;
-        stack = std::max(stack, jobs_[i].scene.stack_depth);
+        stack = std::max(stack, jobs_[i].scene.path_stack_depth);
         bufs[i] = jobBuffers(jobs_[i]);
         accum_bytes += bufs[i].total;
     }
@@ -296,6 +307,17 @@ ab_status Engine::renderBatch(const std::vector<JobSpec>& specs, const std::vect
     work_total_ = total;
     HIP_TRY(hipMemcpyAsync(d_jobs_, jobs_.data(), sizeof(DevJob) * jobs_.size(), hipMemcpyHostToDevice, stream_),
             "hipMemcpy(jobs)");
+    // One segment per job, in job order (the ring holds at least max_jobs entries).
+    HIP_TRY(hipStreamSynchronize(stream_), "hipStreamSynchronize(segments)"); // the mirror is free to rewrite
+    uint64_t begin = 0;
+    for (size_t i = 0; i < jobs_.size(); ++i) {
+        h_segs_[i] = DevSegment{begin, 0, uint32_t(i), 0};
+        begin += jobs_[i].work_count;
+    }
+    seg_lo_ = 0;
+    seg_hi_ = seg_sent_ = uint32_t(jobs_.size());
+    HIP_TRY(hipMemcpyAsync(d_segs_, h_segs_, sizeof(DevSegment) * jobs_.size(), hipMemcpyHostToDevice, stream_),
+            "hipMemcpy(segments)");
     const LaunchShape shape = shapeFor(cfg, capacity, stack);
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0), "hipEventCreate");
@@ -342,6 +364,7 @@ ab_status Engine::beginStream(const ab_launch_config& cfg, uint32_t max_jobs)
     jobs_.clear();
     jobs_.reserve(max_jobs_);
     work_total_ = 0;
+    seg_lo_ = seg_hi_ = seg_sent_ = 0;
     stream_cfg_ = cfg;
     use_graph_ = cfg.use_graph != 0;
     HIP_TRY(launchReset(ps_, 0, 0, stream_), "reset launch");
@@ -358,25 +381,55 @@ ab_status Engine::appendJob(const DevJob& prepared, unsigned long long* accum, u
                             uint32_t* index)
 {
     if (tableFull()) return fail(AB_REJECTED_BUDGET, "job table full");
-    if (prepared.scene.stack_depth > 64)
+    if (prepared.scene.path_stack_depth > 64)
         return fail(AB_INVALID_ARGUMENT, "scene BVH too deep for the wavefront engine (stack > 64)");
     // The bounce kernels follow the deepest scene in the table; queues do not
     // depend on them, so they can change between bounces.
-    if (stackClass(prepared.scene.stack_depth) > shape_.stack) {
-        const ab_status st = setStreamStack(prepared.scene.stack_depth);
+    if (stackClass(prepared.scene.path_stack_depth) > shape_.stack) {
+        const ab_status st = setStreamStack(prepared.scene.path_stack_depth);
         if (st != AB_OK) return st;
     }
     DevJob dj = prepared;
-    dj.work_begin = work_total_;
     dj.accum = accum;
     dj.done = done;
     const uint32_t idx = uint32_t(jobs_.size());
     jobs_.push_back(dj);
     HIP_TRY(hipMemcpyAsync(d_jobs_ + idx, &jobs_[idx], sizeof(DevJob), hipMemcpyHostToDevice, stream_),
             "hipMemcpy(job)");
-    work_total_ += dj.work_count;
-    HIP_TRY(launchAddWork(ps_, work_total_, uint32_t(jobs_.size()), stream_), "add-work launch");
     if (index) *index = idx;
+    return AB_OK;
+}
+
+bool Engine::exposeWork(uint32_t index, uint64_t local, uint64_t count)
+{
+    if (count == 0) return true;
+    if (seg_hi_ - seg_lo_ >= seg_cap_) return false;
+    h_segs_[seg_hi_ & (seg_cap_ - 1)] = DevSegment{work_total_, local, index, 0};
+    ++seg_hi_;
+    work_total_ += count;
+    return true;
+}
+
+ab_status Engine::publishWork(unsigned long long work_next)
+{
+    // Segment seg_lo_ + 1 beginning at or before work_next means seg_lo_ is spent.
+    while (seg_hi_ - seg_lo_ > 1 && h_segs_[(seg_lo_ + 1) & (seg_cap_ - 1)].begin <= work_next) ++seg_lo_;
+    if (seg_sent_ == seg_hi_) return AB_OK;
+    // Copy the new entries, in two runs where they wrap around the ring. The
+    // mirror entries stay untouched until the stream has run these copies: an
+    // entry is rewritten only after it retires, which needs a later counter read.
+    const uint32_t mask = seg_cap_ - 1;
+    uint32_t from = seg_sent_;
+    while (from != seg_hi_) {
+        const uint32_t slot = from & mask;
+        const uint32_t run = std::min<uint32_t>(seg_hi_ - from, seg_cap_ - slot);
+        HIP_TRY(hipMemcpyAsync(d_segs_ + slot, h_segs_ + slot, sizeof(DevSegment) * run, hipMemcpyHostToDevice,
+                               stream_),
+                "hipMemcpy(segments)");
+        from += run;
+    }
+    seg_sent_ = seg_hi_;
+    HIP_TRY(launchAddWork(ps_, work_total_, seg_lo_, seg_hi_, stream_), "add-work launch");
     return AB_OK;
 }
 
@@ -403,6 +456,7 @@ ab_status Engine::resetTable()
 {
     jobs_.clear();
     work_total_ = 0;
+    seg_lo_ = seg_hi_ = seg_sent_ = 0;
     HIP_TRY(launchReset(ps_, 0, 0, stream_), "reset launch");
     return shape_.stack > kShortStack ? setStreamStack(kShortStack) : AB_OK;
 }

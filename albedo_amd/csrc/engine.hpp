@@ -55,10 +55,19 @@ public:
                           const ab_launch_config& cfg);
 
     // Serving interface: jobs are appended while the engine runs. The job table
-    // holds `max_jobs` entries and is emptied by resetTable() once idle.
+    // holds `max_jobs` entries and is emptied by resetTable() once idle. A job's
+    // samples are generated only as the caller exposes them, in segments
+    // (exposeWork, then publishWork), so the caller decides how jobs interleave.
     ab_status beginStream(const ab_launch_config& cfg, uint32_t max_jobs);
     ab_status appendJob(const DevJob& prepared, unsigned long long* accum, unsigned long long* done,
                         uint32_t* index);
+    // Queues work indices [local, local + count) of job `index` after all work
+    // exposed so far. False when the segment ring is full (retry after publishWork).
+    bool exposeWork(uint32_t index, uint64_t local, uint64_t count);
+    // Retires segments generated before `work_next` (the device's counter read
+    // while the stream was idle) and hands the queued segments to the device.
+    ab_status publishWork(unsigned long long work_next);
+    unsigned long long workTotal() const { return work_total_; }
     ab_status cancelJob(uint32_t index);
     ab_status runBounces(uint32_t count); // rounded up to an even count
     ab_status resetTable();
@@ -86,6 +95,11 @@ private:
     PathState ps_{};
     DevJob* d_jobs_{nullptr};
     Counters* h_counters_{nullptr}; // pinned
+    DevSegment* d_segs_{nullptr};   // segment ring on the device
+    DevSegment* h_segs_{nullptr};   // its host mirror (pinned); copies read it when they run
+    uint32_t seg_cap_{0};           // ring entries, a power of two
+    uint32_t seg_lo_{0}, seg_hi_{0}; // live segments, logical indices
+    uint32_t seg_sent_{0};          // segments below this index are on the device
     std::vector<DevJob> jobs_;
     unsigned long long work_total_{0};
     LaunchShape shape_{};          // serving: current bounce shape

@@ -4,8 +4,9 @@
 // (render/render_image.hpp:17-49). A host thread admits submitted jobs whenever
 // their HBM footprint fits the budget, appends them to the running engine's job
 // table (their paths fill free slots on the next bounce, next to the paths of
-// jobs already on the device), retires jobs as their last sample lands and
-// serves progressive snapshots of the per-pixel sums and counts.
+// jobs already on the device), shares generation between the running jobs,
+// retires jobs as their last sample lands and serves progressive snapshots of
+// the per-pixel sums and counts.
 
 #include "context.hpp"
 #include "engine.hpp"
@@ -39,6 +40,7 @@ struct ServerJob {
     JobBuffers buf{};
     void* dev{nullptr}; // accumulators, done counter and mean image (JobBuffers layout)
     uint32_t engine_index{0};
+    uint64_t exposed{0}; // samples handed to the engine so far
     bool cancel_requested{false};
     std::vector<float> result;
 };
@@ -79,12 +81,19 @@ struct Server {
     double busy_ms{0};
     std::vector<double> latencies;
     hipStream_t poll_stream{nullptr};
+    Counters last{};                    // engine counters at the last read (stream idle)
+    unsigned long long last_consumed{0}; // samples generated between the last two reads
+    size_t rr{0};                       // SHARE: rotates which running job is served first
+    uint32_t policy{AB_SCHED_SMALL_FIRST}; // guarded by mu
+    unsigned long long* h_done{nullptr}; // pinned: finished-sample lanes of the running jobs
+    size_t h_done_jobs{0};              // jobs h_done has room for
 
     Server(ab_context* c, uint64_t b) : ctx(c), budget(b), engine(c) {}
 
 
     void loop();
     ab_status admitLocked();
+    ab_status scheduleLocked();
     void retireFinished();
 };
 
@@ -128,22 +137,96 @@ ab_status Server::admitLocked()
     return AB_OK;
 }
 
+// The engine generates work in the order it is exposed, so exposing each job's
+// samples whole on admission would make a job that arrives behind a large one
+// wait for every sample of it. Instead about one window of samples is kept
+// exposed ahead of generation (twice the larger of the path capacity and what
+// the last round of bounces generated, so the device never runs dry within a
+// round) and each round's refill is handed out by the policy:
+//   FIFO: in admission order;
+//   SHARE: split evenly between the jobs with samples left, rotating which
+//          job goes first (processor sharing);
+//   SMALL_FIRST: the jobs with the fewest samples left first (shortest
+//          remaining work first; ties in admission order). Few jobs share the
+//          device at a time, as in FIFO, which keeps the BVHs in flight few.
+// A job's image does not depend on how its work is cut.
+ab_status Server::scheduleLocked()
+{
+    constexpr unsigned long long kMinSlice = 1ull << 16;
+    const unsigned long long cap = engine.capacity();
+    const unsigned long long ahead = engine.workTotal() - last.work_next;
+    const unsigned long long window = std::max<unsigned long long>(2 * cap, 2 * last_consumed);
+    if (ahead < window && !running.empty()) {
+        std::vector<ServerJob*> want; // running jobs with samples left, admission order
+        for (uint64_t id : running) {
+            ServerJob& j = *jobs[id];
+            if (!j.cancel_requested && j.exposed < j.total) want.push_back(&j);
+        }
+        unsigned long long budget = window - ahead;
+        bool room = true;
+        auto expose = [&](ServerJob* j, unsigned long long take) {
+            take = std::min<unsigned long long>(take, j->total - j->exposed);
+            room = engine.exposeWork(j->engine_index, j->exposed, take);
+            if (!room) return;
+            j->exposed += take;
+            budget -= std::min(budget, take);
+        };
+        if (policy == AB_SCHED_SHARE) {
+            if (!want.empty()) std::rotate(want.begin(), want.begin() + rr % want.size(), want.end());
+            rr = rr + 1;
+            while (budget > 0 && room && !want.empty()) {
+                const unsigned long long slice = std::max<unsigned long long>(budget / want.size(), kMinSlice);
+                std::vector<ServerJob*> left;
+                for (ServerJob* j : want) {
+                    if (budget == 0 || !room) break;
+                    expose(j, slice);
+                    if (room && j->exposed < j->total) left.push_back(j);
+                }
+                want.swap(left);
+            }
+        } else {
+            if (policy == AB_SCHED_SMALL_FIRST)
+                std::stable_sort(want.begin(), want.end(), [](const ServerJob* a, const ServerJob* b) {
+                    return a->total - a->exposed < b->total - b->exposed;
+                });
+            for (ServerJob* j : want) {
+                if (budget == 0 || !room) break;
+                expose(j, budget);
+            }
+        }
+    }
+    return engine.publishWork(last.work_next);
+}
+
 void Server::retireFinished()
 {
     // Finished-sample counters of the running jobs, read outside the lock (only
-    // this thread frees job buffers).
+    // this thread frees job buffers): one copy each into pinned memory, one wait.
     std::vector<std::shared_ptr<ServerJob>> live;
     {
         std::lock_guard<std::mutex> lock(mu);
         for (uint64_t id : running) live.push_back(jobs[id]);
     }
-    std::vector<unsigned long long> lanes(kDoneBytes / 8);
-    for (auto& jp : live) {
-        ServerJob& j = *jp;
+    if (live.empty()) return;
+    constexpr size_t kLanesPerJob = kDoneBytes / 8;
+    if (live.size() > h_done_jobs) {
+        const size_t n = std::max<size_t>(live.size(), 2 * h_done_jobs + 16);
+        if (h_done) (void)hipHostFree(h_done);
+        h_done = nullptr;
+        h_done_jobs = 0;
+        if (hipHostMalloc(reinterpret_cast<void**>(&h_done), n * kDoneBytes, hipHostMallocDefault) != hipSuccess)
+            return; // checked again next round
+        h_done_jobs = n;
+    }
+    for (size_t i = 0; i < live.size(); ++i)
+        if (hipMemcpyAsync(h_done + i * kLanesPerJob, static_cast<char*>(live[i]->dev) + live[i]->buf.done_offset,
+                           kDoneBytes, hipMemcpyDeviceToHost, engine.stream()) != hipSuccess)
+            return;
+    if (hipStreamSynchronize(engine.stream()) != hipSuccess) return;
+    for (size_t i = 0; i < live.size(); ++i) {
+        ServerJob& j = *live[i];
         const size_t npx = size_t(j.prepared.width) * j.prepared.height;
-        if (hipMemcpy(lanes.data(), static_cast<char*>(j.dev) + j.buf.done_offset, kDoneBytes,
-                      hipMemcpyDeviceToHost) != hipSuccess)
-            continue;
+        const unsigned long long* lanes = h_done + i * kLanesPerJob;
         unsigned long long d = 0;
         for (uint32_t k = 0; k < kDoneLanes; ++k) d += lanes[size_t(k) * kDoneStride];
         bool cancel = false;
@@ -204,6 +287,11 @@ void Server::loop()
                 cv.wait_for(lock, std::chrono::milliseconds(2), [&] { return stop || !pending.empty(); });
                 continue;
             }
+            const ab_status sst = scheduleLocked();
+            if (sst != AB_OK) {
+                thread_status = sst;
+                break;
+            }
         }
         const double t0 = msSince(ctx);
         Counters c{};
@@ -213,6 +301,8 @@ void Server::loop()
             break;
         }
         busy_ms += msSince(ctx) - t0;
+        last_consumed = c.work_next - last.work_next;
+        last = c;
         rays_now = c.extend_rays + c.shadow_rays;
         retireFinished();
         if (engine.idle(c)) {
@@ -224,6 +314,7 @@ void Server::loop()
                     rays_before += rays_now;
                     rays_now = 0;
                     (void)engine.resetTable();
+                    last = Counters{};
                 }
             }
         }
@@ -292,8 +383,20 @@ ab_status ab_server_stop(ab_context* ctx)
     s->cv.notify_all();
     if (s->thread.joinable()) s->thread.join();
     if (s->poll_stream) (void)hipStreamDestroy(s->poll_stream);
+    if (s->h_done) (void)hipHostFree(s->h_done);
     ctx->server = nullptr;
     delete s;
+    return AB_OK;
+}
+
+ab_status ab_server_set_policy(ab_context* ctx, uint32_t policy)
+{
+    if (!ctx || !ctx->server) return fail(AB_INVALID_ARGUMENT, "ab_server_set_policy: no server running");
+    if (policy > AB_SCHED_SMALL_FIRST)
+        return fail(AB_INVALID_ARGUMENT, "ab_server_set_policy: unknown policy " + std::to_string(policy));
+    Server* s = ctx->server;
+    std::lock_guard<std::mutex> lock(s->mu);
+    s->policy = policy;
     return AB_OK;
 }
 
@@ -308,7 +411,7 @@ ab_status ab_job_submit(ab_context* ctx, const ab_job_desc* job, uint64_t* job_i
         return fail(AB_INVALID_ARGUMENT, "ab_job_submit: scene belongs to another context");
     const ab_status st = prepareJob(j->spec, j->prepared);
     if (st != AB_OK) return st;
-    if (j->prepared.scene.stack_depth > 64)
+    if (j->prepared.scene.path_stack_depth > 64)
         return fail(AB_INVALID_ARGUMENT, "scene BVH too deep for the wavefront engine (stack > 64)");
     j->total = j->prepared.work_count;
     j->buf = jobBuffers(j->prepared);

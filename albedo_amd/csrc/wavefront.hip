@@ -20,15 +20,18 @@ namespace {
 constexpr double kFixedScale = 268435456.0; // 2^28
 constexpr float kMaxSampleRadiance = 1e5f;  // per-sample clamp that keeps 64-bit sums exact
 
-__device__ __forceinline__ uint32_t findJob(const DevJob* jobs, uint32_t n, unsigned long long w)
+// The live segment holding global work index w: the last of [lo, hi) (logical
+// ring indices, hi > lo) whose begin is <= w. Segment lo begins at or before
+// every index still to be generated.
+__device__ __forceinline__ const DevSegment& findSegment(const DevSegment* segs, uint32_t mask, uint32_t lo,
+                                                         uint32_t hi, unsigned long long w)
 {
-    uint32_t lo = 0, hi = n;
     while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (jobs[mid].work_begin <= w) lo = mid;
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if (segs[mid & mask].begin <= w) lo = mid;
         else hi = mid;
     }
-    return lo;
+    return segs[lo & mask];
 }
 
 __device__ __forceinline__ float powerHeuristic(float chosen, float other)
@@ -54,12 +57,14 @@ __global__ void kGenerate(PathState ps)
     const Counters* c = ps.ctr;
     const uint32_t n_new = c->n_new, free_top = c->n_free, act0 = c->n_active;
     const unsigned long long w0 = c->work_next;
+    const uint32_t seg_lo = c->seg_lo, seg_hi = c->seg_hi;
     for (uint32_t i = gridStart(); i < n_new; i += gridStride()) {
         const uint32_t slot = ps.freeq[free_top - 1 - i];
         const unsigned long long w = w0 + i;
-        const uint32_t j = findJob(ps.jobs, c->n_jobs, w);
+        const DevSegment& sg = findSegment(ps.segs, ps.seg_mask, seg_lo, seg_hi, w);
+        const uint32_t j = sg.job;
         const DevJob& job = ps.jobs[j];
-        const unsigned long long local = w - job.work_begin;
+        const unsigned long long local = sg.local + (w - sg.begin);
         const unsigned long long npx = uint64_t(job.width) * job.height;
         const uint32_t group = job.group, spp = job.spa * job.spa;
         const unsigned long long span = npx * group;
@@ -807,7 +812,7 @@ __global__ void kPost(PathState ps)
     c->n_new = uint32_t(left < c->n_free ? left : c->n_free);
 }
 
-__global__ void kReset(PathState ps, unsigned long long total, uint32_t n_jobs)
+__global__ void kReset(PathState ps, unsigned long long total, uint32_t n_segs)
 {
     const uint32_t i0 = gridStart();
     for (uint32_t i = i0; i < ps.capacity; i += gridStride()) ps.freeq[i] = i;
@@ -816,16 +821,18 @@ __global__ void kReset(PathState ps, unsigned long long total, uint32_t n_jobs)
         *c = Counters{};
         c->n_free = ps.capacity;
         c->work_total = total;
-        c->n_jobs = n_jobs;
+        c->seg_lo = 0;
+        c->seg_hi = n_segs;
         c->n_new = uint32_t(total < ps.capacity ? total : ps.capacity);
     }
 }
 
-__global__ void kAddWork(PathState ps, unsigned long long total, uint32_t n_jobs)
+__global__ void kAddWork(PathState ps, unsigned long long total, uint32_t seg_lo, uint32_t seg_hi)
 {
     Counters* c = ps.ctr;
     c->work_total = total;
-    c->n_jobs = n_jobs;
+    c->seg_lo = seg_lo;
+    c->seg_hi = seg_hi;
     const unsigned long long left = total - c->work_next;
     c->n_new = uint32_t(left < c->n_free ? left : c->n_free);
 }
@@ -883,9 +890,9 @@ hipError_t launchBouncePair(const PathState& ps, const LaunchShape& sh, hipStrea
     }
 }
 
-hipError_t launchReset(const PathState& ps, unsigned long long total, uint32_t n_jobs, hipStream_t s)
+hipError_t launchReset(const PathState& ps, unsigned long long total, uint32_t n_segs, hipStream_t s)
 {
-    hipLaunchKernelGGL(kReset, dim3(1024), dim3(256), 0, s, ps, total, n_jobs);
+    hipLaunchKernelGGL(kReset, dim3(1024), dim3(256), 0, s, ps, total, n_segs);
     return hipGetLastError();
 }
 
@@ -898,9 +905,10 @@ hipError_t launchResolve(const unsigned long long* accum, uint64_t npx, float* m
     return hipGetLastError();
 }
 
-hipError_t launchAddWork(const PathState& ps, unsigned long long total, uint32_t n_jobs, hipStream_t s)
+hipError_t launchAddWork(const PathState& ps, unsigned long long total, uint32_t seg_lo, uint32_t seg_hi,
+                         hipStream_t s)
 {
-    hipLaunchKernelGGL(kAddWork, dim3(1), dim3(1), 0, s, ps, total, n_jobs);
+    hipLaunchKernelGGL(kAddWork, dim3(1), dim3(1), 0, s, ps, total, seg_lo, seg_hi);
     return hipGetLastError();
 }
 

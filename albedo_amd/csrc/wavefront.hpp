@@ -21,7 +21,6 @@ struct alignas(16) DevJob {
     uint32_t group;     // samples of one pixel that are consecutive in work order
     uint32_t estimator; // AB_ESTIMATOR_*
     uint64_t seed;
-    uint64_t work_begin; // first global work index of this job
     uint64_t work_count; // width * height * spa^2
     unsigned long long* accum; // 4 per pixel: r, g, b fixed point (2^-28), sample count
     unsigned long long* done;  // samples finished: the sum of kDoneLanes counters (see below)
@@ -35,6 +34,21 @@ constexpr uint32_t kDoneLanes = 64;
 constexpr uint32_t kDoneStride = 32;
 constexpr size_t kDoneBytes = size_t(kDoneLanes) * kDoneStride * 8;
 
+// A run of one job's work placed at global work indices [begin, next segment's
+// begin): global index w is the job's work index local + (w - begin). Generation
+// walks global indices in order, so the order of segments is the order in which
+// jobs' samples start; the serving loop interleaves jobs by appending short
+// segments of each. Segments live in a ring of a power-of-two size, live range
+// [seg_lo, seg_hi) in logical (unwrapped) indices. Images do not depend on how a
+// job's work is cut, since a sample's random numbers depend only on its job,
+// pixel and index.
+struct alignas(8) DevSegment {
+    uint64_t begin;
+    uint64_t local;
+    uint32_t job;
+    uint32_t pad;
+};
+
 // Counters shared by the kernels of one engine.
 struct alignas(16) Counters {
     uint32_t n_active;     // paths queued for extension
@@ -44,8 +58,8 @@ struct alignas(16) Counters {
     uint32_t n_free;
     uint32_t n_new;        // paths generate() creates this round
     uint32_t bin_count[8]; // paths per material bin in the last bounce
-    uint32_t n_jobs;       // entries of the job table in use
-    uint32_t pad;
+    uint32_t seg_lo;       // live segments, logical ring indices [seg_lo, seg_hi)
+    uint32_t seg_hi;
     unsigned long long work_next;
     unsigned long long work_total;
     unsigned long long camera_rays;
@@ -82,8 +96,9 @@ struct PathState {
     uint32_t* freeq;
     Counters* ctr;
     const DevJob* jobs;
+    const DevSegment* segs; // ring of seg_mask + 1 entries
     uint32_t capacity;
-    uint32_t pad;
+    uint32_t seg_mask;
 };
 
 struct LaunchShape {
@@ -97,10 +112,13 @@ struct LaunchShape {
 // `active_next`, the second the other way round, so the queues are back in
 // place afterwards. No host synchronisation; capturable.
 hipError_t launchBouncePair(const PathState& ps, const LaunchShape& shape, hipStream_t s);
-// Initialises counters and the free list for a fresh engine.
-hipError_t launchReset(const PathState& ps, unsigned long long work_total, uint32_t n_jobs, hipStream_t s);
-// Adds newly admitted work to the total (serving).
-hipError_t launchAddWork(const PathState& ps, unsigned long long work_total, uint32_t n_jobs, hipStream_t s);
+// Initialises counters and the free list for a fresh engine whose work is
+// `work_total` samples in segments [0, n_segs) of the ring.
+hipError_t launchReset(const PathState& ps, unsigned long long work_total, uint32_t n_segs, hipStream_t s);
+// Publishes segments appended to the ring: work now ends at `work_total`, live
+// segments are [seg_lo, seg_hi) (serving).
+hipError_t launchAddWork(const PathState& ps, unsigned long long work_total, uint32_t seg_lo, uint32_t seg_hi,
+                         hipStream_t s);
 // Per-pixel mean radiance of a job: sums divided by the sample count, into
 // mean[npx * 3] (zero where no sample landed yet).
 hipError_t launchResolve(const unsigned long long* accum, uint64_t npx, float* mean, hipStream_t s);

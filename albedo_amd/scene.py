@@ -178,28 +178,43 @@ class Sphere:
 
 
 class TriangleMesh:
-    """Positions + triangle faces (+ optional per-face corner uv)."""
+    """Positions + triangle faces (+ optional per-face corner uv). Immutable once
+    built (its arrays are read-only), so derived sets are computed once."""
 
     def __init__(self, positions, faces, uv=None):
-        self.positions = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
-        self.faces = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 3)
-        self.uv = None if uv is None else np.ascontiguousarray(uv, dtype=np.float32).reshape(-1, 6)
+        self.positions = np.array(positions, dtype=np.float32).reshape(-1, 3)
+        self.faces = np.array(faces, dtype=np.uint32).reshape(-1, 3)
+        self.uv = None if uv is None else np.array(uv, dtype=np.float32).reshape(-1, 6)
         if not np.all(np.isfinite(self.positions)):
             raise ValueError("mesh position has a non-finite coordinate")
         if self.faces.size and int(self.faces.max()) >= len(self.positions):
             raise IndexError("mesh face uses a position the mesh does not have")
+        for a in (self.positions, self.faces, self.uv):
+            if a is not None:
+                a.flags.writeable = False
+        self._kept = None
+        self._used = None
 
     def kept_faces(self):
         """Faces whose triangle has a computable area (triangle_mesh.cpp:76-90)."""
-        p = self.positions.astype(np.float64)
-        a, b, c = p[self.faces[:, 0]], p[self.faces[:, 1]], p[self.faces[:, 2]]
-        n = np.cross(b - a, c - a)
-        l2 = (n * n).sum(1)
-        return np.nonzero(np.isfinite(l2) & (l2 >= 2.2250738585072014e-308))[0]
+        if self._kept is None:
+            p = self.positions.astype(np.float64)
+            a, b, c = p[self.faces[:, 0]], p[self.faces[:, 1]], p[self.faces[:, 2]]
+            n = np.cross(b - a, c - a)
+            l2 = (n * n).sum(1)
+            self._kept = np.nonzero(np.isfinite(l2) & (l2 >= 2.2250738585072014e-308))[0]
+            self._kept.flags.writeable = False
+        return self._kept
+
+    def used_positions(self):
+        """Indices of the positions some kept face uses, ascending."""
+        if self._used is None:
+            self._used = np.unique(self.faces[self.kept_faces()].reshape(-1))
+            self._used.flags.writeable = False
+        return self._used
 
     def bounds(self):
-        kept = self.kept_faces()
-        pts = self.positions[self.faces[kept].reshape(-1)].astype(np.float64)
+        pts = self.positions[self.used_positions()].astype(np.float64)
         return pts.min(0), pts.max(0)
 
 
@@ -458,6 +473,5 @@ def fitted(placement: Transform, geometry: MeshGeometry, size: float) -> Transfo
 def grounded(placement: Transform, geometry: MeshGeometry) -> Transform:
     """`placement`, then lifted so its lowest point is at y = 0 (scene_file.cpp grounded())."""
     m = geometry.mesh
-    used = np.unique(m.faces[m.kept_faces()].reshape(-1))
-    ys = placement.apply_points(m.positions[used].astype(np.float64))[:, 1]
+    ys = placement.apply_points(m.positions[m.used_positions()].astype(np.float64))[:, 1]
     return Transform.translation((0.0, -float(ys.min()), 0.0)) * placement

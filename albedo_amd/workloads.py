@@ -316,6 +316,193 @@ def job_mix(n_jobs, seed=3, resolutions=RESOLUTIONS, spas=SPAS, max_spheres=200,
     return jobs
 
 
+# ------------------------------------------------------- displaced icosphere
+
+# Ken Perlin's published permutation ("Improving Noise", SIGGRAPH 2002), the table
+# the reference's noise uses (texture/perlin_noise.cpp:18-32).
+PERLIN_PERMUTATION = np.array([
+    151, 160, 137, 91, 90, 15, 131, 13, 201, 95, 96, 53, 194, 233, 7, 225, 140, 36, 103, 30, 69,
+    142, 8, 99, 37, 240, 21, 10, 23, 190, 6, 148, 247, 120, 234, 75, 0, 26, 197, 62, 94, 252,
+    219, 203, 117, 35, 11, 32, 57, 177, 33, 88, 237, 149, 56, 87, 174, 20, 125, 136, 171, 168,
+    68, 175, 74, 165, 71, 134, 139, 48, 27, 166, 77, 146, 158, 231, 83, 111, 229, 122, 60, 211,
+    133, 230, 220, 105, 92, 41, 55, 46, 245, 40, 244, 102, 143, 54, 65, 25, 63, 161, 1, 216, 80,
+    73, 209, 76, 132, 187, 208, 89, 18, 169, 200, 196, 135, 130, 116, 188, 159, 86, 164, 100,
+    109, 198, 173, 186, 3, 64, 52, 217, 226, 250, 124, 123, 5, 202, 38, 147, 118, 126, 255, 82,
+    85, 212, 207, 206, 59, 227, 47, 16, 58, 17, 182, 189, 28, 42, 223, 183, 170, 213, 119, 248,
+    152, 2, 44, 154, 163, 70, 221, 153, 101, 155, 167, 43, 172, 9, 129, 22, 39, 253, 19, 98,
+    108, 110, 79, 113, 224, 232, 178, 185, 112, 104, 218, 246, 97, 228, 251, 34, 242, 193, 238,
+    210, 144, 12, 191, 179, 162, 241, 81, 51, 145, 235, 249, 14, 239, 107, 49, 192, 214, 31,
+    181, 199, 106, 157, 184, 84, 204, 176, 115, 121, 50, 45, 127, 4, 150, 254, 138, 236, 205,
+    93, 222, 114, 67, 29, 24, 72, 243, 141, 128, 195, 78, 66, 215, 61, 156, 180], dtype=np.int64)
+
+
+def perlin_noise(p):
+    """Improved Perlin noise of points p [n, 3] (FP64), the reference's
+    PerlinNoise::noise (texture/perlin_noise.cpp:70-145) vectorised: lattice cell
+    wrapped to 0..255, quintic fade, the 12-gradient table by hash & 15."""
+    p = np.asarray(p, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros(len(p))
+    ok = np.all(np.isfinite(p), axis=1)
+    q = p[ok]
+    f = np.floor(q)
+    idx = np.mod(f, 256.0).astype(np.int64)
+    x, y, z = (q - f).T
+    xi, yi, zi = idx.T
+    perm = PERLIN_PERMUTATION
+
+    def h(i):
+        return perm[i % 256]
+
+    def fade(t):
+        return t * t * t * (t * (t * 6 - 15) + 10)
+
+    def grad(hash_, gx, gy, gz):
+        hh = hash_ & 15
+        u = np.where(hh < 8, gx, gy)
+        v = np.where(hh < 4, gy, np.where((hh == 12) | (hh == 14), gx, gz))
+        return np.where(hh & 1, -u, u) + np.where(hh & 2, -v, v)
+
+    def lerp(t, a, b):
+        return a + t * (b - a)
+
+    u, v, w = fade(x), fade(y), fade(z)
+    a = h(xi) + yi
+    aa, ab = h(a) + zi, h(a + 1) + zi
+    b = h(xi + 1) + yi
+    ba, bb = h(b) + zi, h(b + 1) + zi
+    out[ok] = lerp(w,
+                   lerp(v, lerp(u, grad(h(aa), x, y, z), grad(h(ba), x - 1, y, z)),
+                        lerp(u, grad(h(ab), x, y - 1, z), grad(h(bb), x - 1, y - 1, z))),
+                   lerp(v, lerp(u, grad(h(aa + 1), x, y, z - 1), grad(h(ba + 1), x - 1, y, z - 1)),
+                        lerp(u, grad(h(ab + 1), x, y - 1, z - 1), grad(h(bb + 1), x - 1, y - 1, z - 1))))
+    return out
+
+
+def fbm(p, octaves=6, lacunarity=2.0, gain=0.5):
+    """Fractional Brownian motion: sum over octaves o of gain^o * noise(lacunarity^o * p)."""
+    p = np.asarray(p, dtype=np.float64).reshape(-1, 3)
+    total = np.zeros(len(p))
+    weight, scale = 1.0, 1.0
+    for _ in range(octaves):
+        total += weight * perlin_noise(p * scale)
+        weight *= gain
+        scale *= lacunarity
+    return total
+
+
+def icosphere(level):
+    """Unit icosphere: the icosahedron with each face split into four `level` times,
+    new vertices pushed onto the sphere. 20 * 4^level faces, 10 * 4^level + 2
+    vertices, faces wound counter-clockwise seen from outside. FP64 positions."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t],
+                  [0, -1, -t], [0, 1, -t], [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], dtype=np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4],
+                  [11, 10, 2], [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8],
+                  [3, 8, 9], [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], dtype=np.int64)
+    for _ in range(level):
+        nv = len(v)
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        key = np.minimum(e[:, 0], e[:, 1]) * nv + np.maximum(e[:, 0], e[:, 1])
+        uniq, inv = np.unique(key, return_inverse=True)
+        lo, hi = uniq // nv, uniq % nv
+        mid = v[lo] + v[hi]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        v = np.concatenate([v, mid])
+        m = (inv.reshape(3, -1) + nv).T  # midpoints of edges (0,1), (1,2), (2,0)
+        a, b, c = f[:, 0], f[:, 1], f[:, 2]
+        m01, m12, m20 = m[:, 0], m[:, 1], m[:, 2]
+        f = np.concatenate([np.stack([a, m01, m20], 1), np.stack([b, m12, m01], 1),
+                            np.stack([c, m20, m12], 1), np.stack([m01, m12, m20], 1)])
+    return v, f
+
+
+def displaced_icosphere(level=8, amplitude=0.15, frequency=2.0, seed=17):
+    """The serving scene's mesh: an icosphere of `level` (level 8: 1,310,720
+    triangles) whose vertices move along the radius to 1 + amplitude * fbm, with
+    6-octave fBm of the reference's Perlin noise evaluated at frequency * p + offset.
+    The reference's permutation is fixed, so the seed picks the offset (PCG32 stream 41)."""
+    v, f = icosphere(level)
+    rng = Pcg32(seed, 41)
+    offset = np.array([rng.uniform(0.0, 256.0) for _ in range(3)])
+    r = 1.0 + amplitude * fbm(v * frequency + offset)
+    return TriangleMesh((v * r[:, None]).astype(np.float32), f.astype(np.uint32))
+
+
+ICOSPHERE_CAMERA = CameraSettings((0.0, 6.5, 15.0), (0.0, 0.4, 0.0), (0.0, 1.0, 0.0), 40.0, 16.0 / 9.0)
+
+
+def icosphere_scene(seed=17, level=8, mesh=None):
+    """64 instances of the displaced icosphere (one shared BLAS) on an 8 x 8 grid over
+    a checker floor (even 0.73, odd 0.3, frequency 1.25): each fit to a seeded size
+    in [0.9, 1.3], turned about a seeded axis, grounded on y = 0 and jittered inside
+    its 1.5-unit cell, with a material drawn from the five room kinds plus white and
+    red diffuse. A spherical area light overhead, a point light and a blue-grey sky."""
+    mesh = mesh if mesh is not None else displaced_icosphere(level, seed=seed)
+    geo = MeshGeometry(mesh, SMOOTH)
+    m = room_materials()
+    kinds = FIVE_KINDS + ("white", "red")
+    s = Scene()
+    floor = MeshGeometry(checker_floor(-7.0, 7.0, -7.0, 7.0), FLAT)
+    s.add(MeshInstance(floor, Transform()),
+          DiffuseMaterial(CheckerTexture((0.73, 0.73, 0.73), (0.3, 0.3, 0.3), 1.25)))
+    rng = Pcg32(seed, 43)
+    for i in range(64):
+        gx, gz = i % 8, i // 8
+        size = rng.uniform(0.9, 1.3)
+        axis = np.array([rng.uniform(-1.0, 1.0), 1.0, rng.uniform(-1.0, 1.0)])
+        angle = rng.uniform(0.0, 360.0)
+        kind = kinds[min(int(rng.real() * len(kinds)), len(kinds) - 1)]
+        jx, jz = rng.uniform(-0.15, 0.15), rng.uniform(-0.15, 0.15)
+        place = fitted(Transform(), geo, size)
+        place = Transform.rotation(axis / np.linalg.norm(axis), angle) * place
+        place = grounded(place, geo)
+        place = Transform.translation(((gx - 3.5) * 1.5 + jx, 0.0, (gz - 3.5) * 1.5 + jz)) * place
+        s.add(MeshInstance(geo, place), m[kind])
+    s.add_area_light(Sphere((0.0, 9.0, 2.0), 1.5), (12.0, 11.0, 9.5))
+    s.add_light(PointLight((-5.0, 6.0, 6.0), (6.0, 6.0, 7.0)))
+    s.set_sky(SkyLight.uniform((0.08, 0.09, 0.12)))
+    return s
+
+
+def serving_library(seed=17, level=8, knot=None):
+    """The 8 scenes the serving workload keeps resident: the displaced-icosphere
+    scene, three sphere-room variants and four full-scene variants holding 5..200
+    random spheres. Returns [(name, scene, camera settings, max_depth)]."""
+    knot = knot if knot is not None else trefoil_mesh()
+    lib = [("icosphere", icosphere_scene(seed, level), ICOSPHERE_CAMERA, 8)]
+    rng = Pcg32(seed, 51)
+    for i in range(7):
+        base = "room" if i < 3 else "full"
+        n_sph = 5 + int(rng.real() * 196)
+        lib.append(("%s+%d" % (base, n_sph), sphere_variant_scene(2000 + i, n_sph, base, knot), ROOM_CAMERA,
+                    8 if base == "room" else 16))
+    return lib
+
+
+def serving_jobs(n_jobs, n_scenes=8, seed=23, resolutions=RESOLUTIONS, spas=SPAS):
+    """Jobs for the serving workload: scene index uniform over the library,
+    resolution uniform over `resolutions` (256x144..1920x1080), samples per axis
+    uniform over `spas` (4..256 spp, perfect squares as stratification needs).
+    Returns [(scene index, width, height, samples per axis, seed)]."""
+    rng = Pcg32(seed, 53)
+    jobs = []
+    for i in range(n_jobs):
+        k = min(int(rng.real() * n_scenes), n_scenes - 1)
+        w, h = resolutions[min(int(rng.real() * len(resolutions)), len(resolutions) - 1)]
+        spa = spas[min(int(rng.real() * len(spas)), len(spas) - 1)]
+        jobs.append((k, w, h, spa, 100 + i))
+    return jobs
+
+
+def poisson_arrivals(n, rate_hz, seed=29):
+    """n arrival times (seconds from 0) of a Poisson process of rate `rate_hz`:
+    exponential gaps by inversion of PCG32 stream 55."""
+    u = pcg32_reals(seed, 55, n)
+    return np.cumsum(-np.log1p(-u) / rate_hz)
+
+
 def render_params(width, height, samples_per_axis, max_depth, roulette_start=3, seed=0,
                   estimator=_abi.ESTIMATOR_PATH):
     p = _abi.RenderParams()

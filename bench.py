@@ -124,31 +124,93 @@ def batch_leg(ctx, n_jobs=16):
             "speedup": round(plain / batched, 3), "tuned_plain": cfg_plain, "tuned_batched": cfg_batch}
 
 
-def serve_leg(ctx, n_jobs=24, rate_hz=40.0):
-    """Poisson arrivals of mixed jobs into the serving loop; p50/p99 latency."""
+def _fifo_latencies(arrivals, service):
+    """Latencies of a one-job-at-a-time server taking jobs in arrival order."""
+    free, out = 0.0, []
+    for a, s in zip(arrivals, service):
+        free = max(free, a) + s
+        out.append(free - a)
+    return np.array(out)
+
+
+def serve_leg(ctx, n_jobs=24, loads=(0.5, 1.0, 2.0), budget=1 << 30):
+    """Serving over a library of 8 HBM-resident scenes (the 1,310,720-triangle
+    displaced icosphere x 64 instances on a checker floor, three sphere-room and
+    four full-scene variants with 5..200 random spheres). Jobs: random scene,
+    resolution 256x144..1920x1080, 4..256 spp. Poisson arrivals at rates swept as
+    multiples of the plain path's measured capacity; admission by a 1 GiB HBM
+    budget. The plain path (each job alone, one at a time, in arrival order) is
+    timed on the same jobs and replayed against the same arrival times."""
+    import hashlib
     import albedo_amd
+    from albedo_amd import _abi
     from albedo_amd import workloads as W
     from albedo_amd.scene import CameraSettings
-    knot = W.trefoil_mesh()
-    mix = W.job_mix(n_jobs, seed=5, resolutions=W.RESOLUTIONS[:3], spas=W.SPAS[:3], knot=knot)
-    c = W.ROOM_CAMERA
-    scenes = [ctx.scene(s) for s, *_ in mix]
-    rng = np.random.default_rng(9)
-    gaps = rng.exponential(1.0 / rate_hz, n_jobs)
     t0 = time.perf_counter()
-    with albedo_amd.Server(ctx, 8 << 30) as srv:
-        ids = []
-        for (scene, (w, h), spa, depth, seed), g, gap in zip(mix, scenes, gaps):
-            time.sleep(gap)
-            ids.append(srv.submit(g, CameraSettings(c.look_from, c.look_at, (0, 1, 0), 38.0, w / h), w, h, spa,
-                                  depth, 3, seed))
-        for i in ids:
-            srv.wait(i)
-        wall = time.perf_counter() - t0
-        st = srv.stats()
-    return {"workload": "%d mixed jobs, Poisson arrivals at %.0f/s" % (n_jobs, rate_hz),
-            "p50_latency_ms": round(st["p50_latency_ms"], 2), "p99_latency_ms": round(st["p99_latency_ms"], 2),
-            "mrays_s": round(st["rays"] / wall / 1e6, 1), "wall_s": round(wall, 3)}
+    library = W.serving_library()
+    t_gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    gscenes = [ctx.scene(s) for _, s, _, _ in library]
+    t_load = time.perf_counter() - t0
+    print("serve: library generated in %.1f s, built and resident in %.1f s" % (t_gen, t_load), file=sys.stderr,
+          flush=True)
+
+    def job_args(k, w, h, spa, seed):
+        _, _, c, depth = library[k]
+        cam = CameraSettings(c.look_from, c.look_at, (0, 1, 0), c.vertical_fov_degrees, w / h)
+        return (gscenes[k], cam, w, h, spa, depth, 3, seed)
+
+    jobs = [job_args(*j) for j in W.serving_jobs(n_jobs, len(library))]
+    for k, g in enumerate(gscenes):  # first use of each scene's launch shapes
+        g.render(*job_args(k, 64, 36, 1, 0)[1:])
+    service, digests, rays_plain = [], [], 0
+    for j in jobs:
+        t1 = time.perf_counter()
+        img, st = j[0].render(*j[1:], stats=True)
+        service.append(time.perf_counter() - t1)
+        rays_plain += st["camera_rays"] + st["extension_rays"] + st["shadow_rays"]
+        digests.append(hashlib.sha1(img.tobytes()).hexdigest())
+    capacity = n_jobs / sum(service)
+    samples = sum(j[2] * j[3] * j[4] ** 2 for j in jobs)
+    print("serve: plain path %.2f s for %d jobs" % (sum(service), n_jobs), file=sys.stderr, flush=True)
+    sweep = []
+    names = {_abi.SCHED_FIFO: "fifo", _abi.SCHED_SHARE: "share", _abi.SCHED_SMALL_FIRST: "small_first"}
+    runs = [(load, _abi.SCHED_SMALL_FIRST) for load in loads]
+    runs += [(1.0, p) for p in (_abi.SCHED_FIFO, _abi.SCHED_SHARE)]
+    for load, policy in runs:
+        rate = load * capacity
+        arrivals = W.poisson_arrivals(n_jobs, rate)
+        with albedo_amd.Server(ctx, budget, policy=policy) as srv:
+            t0 = time.perf_counter()
+            ids = []
+            for j, at in zip(jobs, arrivals):
+                delay = t0 + at - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                ids.append(srv.submit(*j))
+            same = 0
+            for i, d in zip(ids, digests):
+                _, _, img = srv.wait(i)
+                same += hashlib.sha1(img.tobytes()).hexdigest() == d
+            wall = time.perf_counter() - t0
+            st = srv.stats()
+        plain_lat = _fifo_latencies(arrivals, service) * 1e3
+        plain_wall = float(np.max(arrivals + plain_lat / 1e3))  # last result, from time 0 as `wall`
+        sweep.append({"rate_jobs_s": round(rate, 3), "load": load, "policy": names[policy],
+                      "p50_latency_ms": round(st["p50_latency_ms"], 1),
+                      "p99_latency_ms": round(st["p99_latency_ms"], 1),
+                      "mrays_s": round(st["rays"] / wall / 1e6, 1), "wall_s": round(wall, 2),
+                      "plain_p50_latency_ms": round(float(np.percentile(plain_lat, 50)), 1),
+                      "plain_p99_latency_ms": round(float(np.percentile(plain_lat, 99)), 1),
+                      "plain_mrays_s": round(rays_plain / plain_wall / 1e6, 1),
+                      "results_equal_alone": "%d/%d" % (same, n_jobs)})
+        print("serve: load %.2f %s done in %.1f s" % (load, names[policy], wall), file=sys.stderr, flush=True)
+    return {"workload": "%d jobs over 8 resident scenes (1,310,720-triangle displaced icosphere x 64 instances, "
+                        "room and full-scene variants), 256x144..1920x1080, 4..256 spp, Poisson arrivals, "
+                        "1 GiB HBM budget" % n_jobs,
+            "scenes": [name for name, *_ in library], "library_build_s": round(t_gen + t_load, 1),
+            "samples": samples, "plain_capacity_jobs_s": round(capacity, 3),
+            "plain_mrays_s_busy": round(rays_plain / sum(service) / 1e6, 1), "sweep": sweep}
 
 
 def main():

@@ -311,6 +311,20 @@ ab_status ab_autotune(ab_context* ctx, const ab_job_desc* jobs, uint32_t n_jobs,
  * budget allow, renders them together, and publishes progressive results. */
 ab_status ab_server_start(ab_context* ctx, uint64_t hbm_budget_bytes);
 ab_status ab_server_stop(ab_context* ctx);
+/* Order in which the running jobs' samples are started (images never depend on
+ * it). The loop keeps about two rounds of bounces' worth of samples handed to the
+ * device and refills that window every round:
+ *   AB_SCHED_FIFO          in admission order: the least worst-case latency;
+ *   AB_SCHED_SHARE         equal shares for every running job (processor sharing):
+ *                          small jobs finish fast, large ones stretch;
+ *   AB_SCHED_SMALL_FIRST   the jobs with the fewest samples left first (shortest
+ *                          remaining work first): a low median latency with few
+ *                          jobs sharing the device at a time (the default).
+ * Takes effect at the next round. No counterpart in the reference. */
+#define AB_SCHED_FIFO 0u
+#define AB_SCHED_SHARE 1u
+#define AB_SCHED_SMALL_FIRST 2u
+ab_status ab_server_set_policy(ab_context* ctx, uint32_t policy);
 ab_status ab_job_submit(ab_context* ctx, const ab_job_desc* job, uint64_t* job_id);
 /* HBM bytes a job holds while admitted (accumulators, counter, mean image): the
  * amount ab_job_submit charges against the budget. Validates the job like

@@ -26,3 +26,13 @@ def pytest_runtest_setup(item):
     # still says which test was running.
     if "gpu" in item.keywords:
         print("\n[gpu test] %s" % item.nodeid, flush=True)
+
+
+@pytest.fixture(scope="session")
+def icosphere(ctx):
+    """The serving workload's 1,310,720-triangle displaced icosphere x 64 instances:
+    (scene, device scene, oracle scene), built once per session."""
+    import oracle
+    from albedo_amd import workloads as W
+    s = W.icosphere_scene()
+    return s, ctx.scene(s), oracle.OracleScene(s.to_desc())

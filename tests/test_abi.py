@@ -35,6 +35,17 @@ def test_struct_sizes(struct, size):
     assert ctypes.sizeof(struct) == size
 
 
+def test_header_constants_match_the_mirror():
+    text = open(os.path.join(ROOT, "include", "albedo_gpu.h")).read()
+    defines = {k: int(v) for k, v in re.findall(r"#define (AB_[A-Z_0-9]+) (\d+)u", text)}
+    mirror = {"AB_TUNE_PLAIN": _abi.AB_TUNE_PLAIN, "AB_TUNE_BATCH": _abi.AB_TUNE_BATCH,
+              "AB_ESTIMATOR_PATH": _abi.ESTIMATOR_PATH, "AB_ESTIMATOR_SCATTER_ONLY": _abi.ESTIMATOR_SCATTER_ONLY,
+              "AB_ESTIMATOR_NORMALS": _abi.ESTIMATOR_NORMALS, "AB_SCHED_FIFO": _abi.SCHED_FIFO,
+              "AB_SCHED_SHARE": _abi.SCHED_SHARE, "AB_SCHED_SMALL_FIRST": _abi.SCHED_SMALL_FIRST}
+    for name, value in mirror.items():
+        assert defines[name] == value, name
+
+
 def test_abi_version():
     assert _abi.lib().ab_abi_version() == 1
 

@@ -101,3 +101,46 @@ def test_progressive_snapshot_converges_to_final(ctx, scenes):
             assert np.all(np.isfinite(partial))
             rel = abs(float(partial.mean()) - float(final.mean())) / float(final.mean())
             assert rel < 0.5
+
+
+def test_mix_with_million_triangle_scene_equals_alone(ctx, scenes, icosphere):
+    # Jobs of the sphere room (20-entry stack class) are in flight when jobs of the
+    # 1.3M-triangle instanced scene (64-entry class) arrive and upgrade the shared
+    # launch; each result still equals its render alone.
+    room, full = scenes
+    _, ico, _ = icosphere
+    c = W.ICOSPHERE_CAMERA
+    ico_cam = CameraSettings(c.look_from, c.look_at, (0, 1, 0), c.vertical_fov_degrees, 96 / 54)
+    jobs = [(room, cam(160 / 90), 160, 90, 4, 8, 3, 21), (ico, ico_cam, 96, 54, 3, 8, 3, 22),
+            (full, cam(64 / 36), 64, 36, 2, 16, 3, 23), (ico, ico_cam, 96, 54, 2, 8, 3, 24),
+            (room, cam(1.0), 24, 24, 2, 8, 3, 25)]
+    with Server(ctx, 1 << 30) as srv:
+        ids = [srv.submit(*j) for j in jobs]
+        results = [srv.wait(i) for i in ids]
+    for (g, cm, w, h, n, d, rr, seed), (code, status, img) in zip(jobs, results):
+        assert code == _abi.AB_OK and status["samples_done"] == w * h * n * n
+        alone = g.render(cm, w, h, n, d, rr, seed)
+        assert np.array_equal(alone.view(np.uint32), img.view(np.uint32))
+
+
+@pytest.mark.parametrize("policy", [_abi.SCHED_FIFO, _abi.SCHED_SHARE, _abi.SCHED_SMALL_FIRST])
+def test_policies_keep_images_and_order_jobs(ctx, scenes, policy):
+    room, full = scenes
+    big = (full, cam(16 / 9), 640, 360, 16, 16, 3, 31)    # 59M samples
+    small = (room, cam(1.0), 32, 32, 2, 8, 3, 32)          # 4096 samples
+    with Server(ctx, 1 << 30, policy=policy) as srv:
+        a = srv.submit(*big)
+        b = srv.submit(*small)
+        _, sa, ia = srv.wait(a)
+        _, sb, ib = srv.wait(b)
+    assert np.array_equal(ia.view(np.uint32), full.render(*big[1:]).view(np.uint32))
+    assert np.array_equal(ib.view(np.uint32), room.render(*small[1:]).view(np.uint32))
+    if policy != _abi.SCHED_FIFO:
+        # The small job does not wait for the large one's samples.
+        assert sb["finish_ms"] < sa["finish_ms"]
+
+
+def test_unknown_policy_is_rejected(ctx):
+    with Server(ctx, 1 << 30) as srv:
+        with pytest.raises(AlbedoError, match="unknown policy"):
+            srv.set_policy(7)

@@ -212,3 +212,25 @@ def test_device_pointer_entry_point_and_streams(ctx, trefoil):
     s.synchronize()
     host = d_hits.cpu().numpy().view(np.uint32)
     assert np.array_equal(host, g.trace_closest(rays))
+
+
+def test_million_triangle_instanced_scene_bit_exact(icosphere):
+    # 64 instances of a 1,310,720-triangle BLAS: the deepest stacks the library
+    # traces (TLAS depth 9 + BLAS depth 25, the 64-entry stack class).
+    s, g, o = icosphere
+    rng = np.random.default_rng(12)
+    n = 1 << 17
+    org = np.stack([rng.uniform(-7, 7, n), rng.uniform(0.05, 4.0, n), rng.uniform(-7, 7, n)], 1)
+    org[: n // 2] = (0.0, 6.5, 15.0)  # half from the serving camera, aimed at the grid
+    tgt = np.stack([rng.uniform(-6, 6, n), rng.uniform(0.0, 1.5, n), rng.uniform(-6, 6, n)], 1)
+    d = np.where(np.arange(n)[:, None] < n // 2, tgt - org, rng.normal(size=(n, 3)))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = org
+    rays[:, 4:7] = d
+    rays[:, 7] = np.inf
+    hits = g.trace_closest(rays)
+    assert_bit_exact(hits, o.trace_closest(rays, fp32=True), "icosphere instances")
+    assert len(np.unique(hits[:, 1])) > 60  # the floor and nearly every instance
+    occ = g.trace_any(rays)
+    assert np.array_equal(occ, o.trace_any(rays, fp32=True))
