@@ -430,25 +430,44 @@ struct RayQuery {
 
     __device__ __forceinline__ void step(const DevScene& sc, LdsStack<BLOCK> stk)
     {
-        if (ref >= 0) {
-            const DevNode n = loadNode(sc.nodes, ref);
-            bool h0, h1;
-            float t0, t1;
-            const float lim = ANY ? cullLimit(r.tmax) : cullLimit(best.object == kMissId ? r.tmax : best.t);
-            testChildren(n, box, r.tmin, lim, h0, h1, t0, t1);
-            if (h0 && h1) {
-                const bool swap = t1 < t0;
-                stk.put(sp++, swap ? n.child[0] : n.child[1]);
-                ref = swap ? n.child[1] : n.child[0];
-                return;
-            }
-            if (h0 || h1) {
-                ref = h0 ? n.child[0] : n.child[1];
-                return;
-            }
-            pop(stk);
+        if (ref >= 0) visitNode(sc, stk);
+        else visitLeaf(sc, stk);
+    }
+
+    // Node visits until every lane of the wave still tracing holds a leaf, then
+    // that leaf: the leaf tests (the longer code) run with the wave's lanes
+    // together instead of interleaved with other lanes' node visits ("while-while",
+    // Aila & Laine 2009). Visits the same nodes and leaves in the same order as step().
+    __device__ __forceinline__ void stepLeaf(const DevScene& sc, LdsStack<BLOCK> stk)
+    {
+        while (__ballot(!done && ref >= 0)) {
+            if (!done && ref >= 0) visitNode(sc, stk);
+        }
+        if (!done) visitLeaf(sc, stk);
+    }
+
+    __device__ __forceinline__ void visitNode(const DevScene& sc, LdsStack<BLOCK> stk)
+    {
+        const DevNode n = loadNode(sc.nodes, ref);
+        bool h0, h1;
+        float t0, t1;
+        const float lim = ANY ? cullLimit(r.tmax) : cullLimit(best.object == kMissId ? r.tmax : best.t);
+        testChildren(n, box, r.tmin, lim, h0, h1, t0, t1);
+        if (h0 && h1) {
+            const bool swap = t1 < t0;
+            stk.put(sp++, swap ? n.child[0] : n.child[1]);
+            ref = swap ? n.child[1] : n.child[0];
             return;
         }
+        if (h0 || h1) {
+            ref = h0 ? n.child[0] : n.child[1];
+            return;
+        }
+        pop(stk);
+    }
+
+    __device__ __forceinline__ void visitLeaf(const DevScene& sc, LdsStack<BLOCK> stk)
+    {
         const uint32_t enc = uint32_t(~ref);
         const uint32_t first = enc >> kLeafCountBits, count = enc & ((1u << kLeafCountBits) - 1);
         if (in_blas) {
