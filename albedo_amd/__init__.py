@@ -210,8 +210,11 @@ class Server:
         d.params = render_params(width, height, samples_per_axis, max_depth, roulette_start, seed)
         return d
 
-    def submit(self, scene, camera, width, height, samples_per_axis, max_depth=64, roulette_start=3, seed=0):
+    def submit(self, scene, camera, width, height, samples_per_axis, max_depth=64, roulette_start=3, seed=0,
+               keep_sums=False):
+        """keep_sums: the finished job still answers poll_sums (AB_JOB_KEEP_SUMS)."""
         d = self.job_desc(scene, camera, width, height, samples_per_axis, max_depth, roulette_start, seed)
+        d.flags = _abi.JOB_KEEP_SUMS if keep_sums else 0
         jid = ctypes.c_uint64()
         self.ctx.check(self.ctx.lib.ab_job_submit(self.ctx.handle, ctypes.byref(d), ctypes.byref(jid)))
         self.shapes[jid.value] = (height, width)
@@ -228,6 +231,17 @@ class Server:
     def _status(self, st):
         return {"state": st.state, "samples_done": st.samples_done, "samples_total": st.samples_total,
                 "submit_ms": st.submit_ms, "start_ms": st.start_ms, "finish_ms": st.finish_ms}
+
+    def poll_sums(self, jid):
+        """(status code, status dict, sums uint64 [h, w, 4]): the wire format of
+        ab_job_poll_sums (red, green, blue fixed-point sums and the sample count)."""
+        h, w = self.shapes[jid]
+        out = np.zeros((h, w, 4), np.uint64)
+        st = _abi.JobStatus()
+        code = self.ctx.lib.ab_job_poll_sums(self.ctx.handle, jid, ctypes.byref(st), out.ctypes.data)
+        if code not in (_abi.AB_OK, _abi.AB_PENDING, _abi.AB_CANCELLED):
+            self.ctx.check(code)
+        return code, self._status(st), out
 
     def poll(self, jid, image=True):
         """(status code, status dict, image or None)."""
@@ -263,6 +277,24 @@ class Server:
         self.ctx.check(self.ctx.lib.ab_server_stats_get(self.ctx.handle, ctypes.byref(s)))
         return {"jobs_done": s.jobs_done, "jobs_rejected": s.jobs_rejected, "rays": s.rays, "busy_ms": s.busy_ms,
                 "p50_latency_ms": s.p50_latency_ms, "p99_latency_ms": s.p99_latency_ms}
+
+
+def sums_resolve(sums: np.ndarray) -> np.ndarray:
+    """Per-pixel means of wire-format sums [..., 4] (ab_sums_resolve), float32 [..., 3]."""
+    s = np.ascontiguousarray(sums, dtype=np.uint64)
+    npx = s.size // 4
+    out = np.zeros(s.shape[:-1] + (3,), np.float32)
+    check(lib().ab_sums_resolve(s.ctypes.data, npx, out.ctypes.data))
+    return out
+
+
+def sums_merge(dst: np.ndarray, src: np.ndarray) -> np.ndarray:
+    """dst += src over wire-format sums (ab_sums_merge), in place; returns dst."""
+    if dst.dtype != np.uint64 or not dst.flags.c_contiguous or dst.shape != src.shape:
+        raise ValueError("sums_merge needs two C-contiguous uint64 arrays of one shape")
+    s = np.ascontiguousarray(src, dtype=np.uint64)
+    check(lib().ab_sums_merge(dst.ctypes.data, s.ctypes.data, dst.size // 4))
+    return dst
 
 
 def render_batch(ctx: Context, jobs):

@@ -29,6 +29,8 @@ STATUS_NAMES = {0: "AB_OK", 1: "AB_INVALID_ARGUMENT", 2: "AB_OUT_OF_MEMORY", 3: 
 AB_TUNE_PLAIN, AB_TUNE_BATCH = 0, 1
 ESTIMATOR_PATH, ESTIMATOR_SCATTER_ONLY, ESTIMATOR_NORMALS = 0, 1, 2
 SCHED_FIFO, SCHED_SHARE, SCHED_SMALL_FIRST = 0, 1, 2
+JOB_KEEP_SUMS = 1
+SUMS_FRACTION_BITS = 28
 
 AB_MISS = 0xFFFFFFFF
 AB_NO_PRIM = 0xFFFFFFFF
@@ -106,7 +108,7 @@ class LaunchConfig(ctypes.Structure):
 
 
 class JobDesc(ctypes.Structure):
-    _fields_ = [("scene", ctypes.c_void_p), ("camera", Camera), ("params", RenderParams)]
+    _fields_ = [("scene", ctypes.c_void_p), ("camera", Camera), ("params", RenderParams), ("flags", ctypes.c_uint32)]
 
 
 class JobStatus(ctypes.Structure):
@@ -152,6 +154,9 @@ PROTOTYPES = {
     "ab_job_footprint": (_S, [ctypes.POINTER(JobDesc), ctypes.POINTER(ctypes.c_uint64)]),
     "ab_job_poll": (_S, [_P, ctypes.c_uint64, ctypes.POINTER(JobStatus), _P]),
     "ab_job_wait": (_S, [_P, ctypes.c_uint64, ctypes.POINTER(JobStatus), _P]),
+    "ab_job_poll_sums": (_S, [_P, ctypes.c_uint64, ctypes.POINTER(JobStatus), _P]),
+    "ab_sums_merge": (_S, [_P, _P, ctypes.c_uint64]),
+    "ab_sums_resolve": (_S, [_P, ctypes.c_uint64, _P]),
     "ab_job_cancel": (_S, [_P, ctypes.c_uint64]),
     "ab_job_release": (_S, [_P, ctypes.c_uint64]),
     "ab_server_stats_get": (_S, [_P, ctypes.POINTER(ServerStats)]),

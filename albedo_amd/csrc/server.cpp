@@ -41,9 +41,20 @@ struct ServerJob {
     void* dev{nullptr}; // accumulators, done counter and mean image (JobBuffers layout)
     uint32_t engine_index{0};
     uint64_t exposed{0}; // samples handed to the engine so far
+    uint32_t flags{0};   // AB_JOB_*
     bool cancel_requested{false};
     std::vector<float> result;
+    std::vector<uint64_t> sums; // final accumulators, with AB_JOB_KEEP_SUMS
 };
+
+// Copies a job's accumulators (4 words per pixel) to host memory on `s`.
+hipError_t readSums(const ServerJob& j, uint64_t* out, hipStream_t s)
+{
+    const size_t npx = size_t(j.prepared.width) * j.prepared.height;
+    hipError_t e = hipMemcpyAsync(out, j.dev, npx * 32, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e;
+}
 
 // Resolves a job's accumulators into its mean image on `s` and copies it to host.
 hipError_t readMean(const ServerJob& j, float* out, hipStream_t s)
@@ -253,7 +264,11 @@ void Server::retireFinished()
         // buffer meanwhile.
         std::lock_guard<std::mutex> lock(mu);
         std::vector<float> result(npx * 3);
-        const hipError_t e = readMean(j, result.data(), engine.stream());
+        hipError_t e = readMean(j, result.data(), engine.stream());
+        if (e == hipSuccess && (j.flags & AB_JOB_KEEP_SUMS)) {
+            j.sums.resize(npx * 4);
+            e = readSums(j, j.sums.data(), engine.stream());
+        }
         if (e == hipSuccess) {
             j.result = std::move(result);
             j.state = kDone;
@@ -413,6 +428,9 @@ ab_status ab_job_submit(ab_context* ctx, const ab_job_desc* job, uint64_t* job_i
     if (st != AB_OK) return st;
     if (j->prepared.scene.path_stack_depth > 64)
         return fail(AB_INVALID_ARGUMENT, "scene BVH too deep for the wavefront engine (stack > 64)");
+    if (job->flags & ~AB_JOB_KEEP_SUMS)
+        return fail(AB_INVALID_ARGUMENT, "ab_job_submit: unknown job flags " + std::to_string(job->flags));
+    j->flags = job->flags;
     j->total = j->prepared.work_count;
     j->buf = jobBuffers(j->prepared);
     j->footprint = j->buf.total;
@@ -482,6 +500,58 @@ ab_status ab_job_poll(ab_context* ctx, uint64_t id, ab_job_status* status, float
     if (j.state == kCancelled) return AB_CANCELLED;
     if (j.state == kFailed) return fail(AB_DEVICE_ERROR, "job failed");
     return j.state == kDone ? AB_OK : AB_PENDING;
+}
+
+ab_status ab_job_poll_sums(ab_context* ctx, uint64_t id, ab_job_status* status, uint64_t* sums)
+{
+    if (!ctx || !ctx->server) return fail(AB_INVALID_ARGUMENT, "ab_job_poll_sums: no server running");
+    if (!sums) return fail(AB_INVALID_ARGUMENT, "ab_job_poll_sums: null output");
+    Server* s = ctx->server;
+    std::lock_guard<std::mutex> lock(s->mu);
+    auto it = s->jobs.find(id);
+    if (it == s->jobs.end()) return fail(AB_NOT_FOUND, "ab_job_poll_sums: unknown job " + std::to_string(id));
+    ServerJob& j = *it->second;
+    fillStatus(j, status);
+    const size_t npx = size_t(j.prepared.width) * j.prepared.height;
+    if (j.state == kDone) {
+        if (j.sums.size() != npx * 4)
+            return fail(AB_INVALID_ARGUMENT, "ab_job_poll_sums: job " + std::to_string(id) +
+                                                 " finished without AB_JOB_KEEP_SUMS");
+        std::copy(j.sums.begin(), j.sums.end(), sums);
+        return AB_OK;
+    }
+    if (j.state == kRunning && j.dev) {
+        (void)hipSetDevice(ctx->device);
+        if (!s->poll_stream && hipStreamCreateWithFlags(&s->poll_stream, hipStreamNonBlocking) != hipSuccess)
+            return fail(AB_DEVICE_ERROR, "ab_job_poll_sums: stream creation failed");
+        const hipError_t e = readSums(j, sums, s->poll_stream);
+        if (e != hipSuccess) return hipFail(e, "ab_job_poll_sums: snapshot");
+    } else {
+        std::fill(sums, sums + npx * 4, uint64_t(0));
+    }
+    if (j.state == kCancelled) return AB_CANCELLED;
+    if (j.state == kFailed) return fail(AB_DEVICE_ERROR, "job failed");
+    return AB_PENDING;
+}
+
+ab_status ab_sums_merge(uint64_t* dst, const uint64_t* src, uint64_t pixels)
+{
+    if (pixels && (!dst || !src)) return fail(AB_INVALID_ARGUMENT, "ab_sums_merge: null buffer");
+    for (uint64_t i = 0; i < 4 * pixels; ++i) dst[i] += src[i];
+    return AB_OK;
+}
+
+ab_status ab_sums_resolve(const uint64_t* sums, uint64_t pixels, float* rgb)
+{
+    if (pixels && (!sums || !rgb)) return fail(AB_INVALID_ARGUMENT, "ab_sums_resolve: null buffer");
+    // The device's resolve (kResolve): sum / 2^28 / count in FP64, rounded once to FP32.
+    const double scale = double(uint64_t(1) << AB_SUMS_FRACTION_BITS);
+    for (uint64_t p = 0; p < pixels; ++p) {
+        const uint64_t* a = sums + 4 * p;
+        const double n = double(a[3]);
+        for (int c = 0; c < 3; ++c) rgb[3 * p + c] = a[3] ? float(double(a[c]) / scale / n) : 0.0f;
+    }
+    return AB_OK;
 }
 
 ab_status ab_job_wait(ab_context* ctx, uint64_t id, ab_job_status* status, float* rgb)

@@ -18,6 +18,7 @@ namespace ab {
 namespace {
 
 constexpr double kFixedScale = 268435456.0; // 2^28
+static_assert(kFixedScale == double(1ull << AB_SUMS_FRACTION_BITS), "sums fixed point = the wire format's");
 constexpr float kMaxSampleRadiance = 1e5f;  // per-sample clamp that keeps 64-bit sums exact
 
 // The live segment holding global work index w: the last of [lo, hi) (logical

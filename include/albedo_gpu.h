@@ -280,7 +280,12 @@ typedef struct ab_job_desc {
     ab_scene* scene;
     ab_camera camera;
     ab_render_params params;
+    uint32_t flags;          /* AB_JOB_* (serving only; ignored by ab_render_batch) */
 } ab_job_desc;
+
+/* The server keeps a finished job's per-pixel sums (ab_job_poll_sums) until the
+ * job is released, in host memory (32 B per pixel). */
+#define AB_JOB_KEEP_SUMS 1u
 
 typedef struct ab_job_status {
     uint32_t state;          /* 0 queued, 1 running, 2 done, 3 cancelled, 4 failed */
@@ -332,6 +337,21 @@ ab_status ab_job_submit(ab_context* ctx, const ab_job_desc* job, uint64_t* job_i
 ab_status ab_job_footprint(const ab_job_desc* job, uint64_t* bytes);
 /* Progressive snapshot: the mean over the samples finished so far. */
 ab_status ab_job_poll(ab_context* ctx, uint64_t job_id, ab_job_status* status, float* rgb_mean);
+
+/* Progressive results as exact sums (the wire format). Per pixel, row-major from
+ * the top-left corner, four little-endian 64-bit words: the red, green and blue
+ * radiance sums as unsigned fixed point (value = word / 2^AB_SUMS_FRACTION_BITS)
+ * and the sample count -- the reference's per-pixel accumulator of sum and count
+ * (image/pixel_buffer.hpp:54-57) in integer form. Sums of disjoint sample sets
+ * combine exactly, in any order, by adding words (ab_sums_merge), and
+ * ab_sums_resolve gives the same means as ab_job_poll / ab_job_wait. A snapshot
+ * of a running job may catch a pixel between its sum and its count update; the
+ * final one (after ab_job_wait) is exact. A finished job answers only if it was
+ * submitted with AB_JOB_KEEP_SUMS. No counterpart in the reference. */
+#define AB_SUMS_FRACTION_BITS 28
+ab_status ab_job_poll_sums(ab_context* ctx, uint64_t job_id, ab_job_status* status, uint64_t* sums);
+ab_status ab_sums_merge(uint64_t* dst, const uint64_t* src, uint64_t pixels);
+ab_status ab_sums_resolve(const uint64_t* sums, uint64_t pixels, float* rgb_mean);
 ab_status ab_job_wait(ab_context* ctx, uint64_t job_id, ab_job_status* status, float* rgb_mean);
 ab_status ab_job_cancel(ab_context* ctx, uint64_t job_id);
 ab_status ab_job_release(ab_context* ctx, uint64_t job_id);

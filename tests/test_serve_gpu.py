@@ -6,6 +6,7 @@ import time
 import numpy as np
 import pytest
 
+import albedo_amd
 from albedo_amd import AlbedoError, Server, _abi
 from albedo_amd import workloads as W
 from albedo_amd.scene import CameraSettings
@@ -144,3 +145,31 @@ def test_unknown_policy_is_rejected(ctx):
     with Server(ctx, 1 << 30) as srv:
         with pytest.raises(AlbedoError, match="unknown policy"):
             srv.set_policy(7)
+
+
+def test_wire_format_sums_are_progressive_and_exact(ctx, scenes):
+    room, _ = scenes
+    job = (room, cam(16 / 9), 160, 90, 8, 8, 3, 41)
+    with Server(ctx, 1 << 30) as srv:
+        j = srv.submit(*job, keep_sums=True)
+        k = srv.submit(*job)  # the same job without kept sums
+        counts = []
+        for _ in range(400):
+            code, st, sums = srv.poll_sums(j)
+            counts.append(sums[..., 3].astype(np.int64))
+            if code == _abi.AB_OK:
+                break
+            time.sleep(0.002)
+        code, st, img = srv.wait(j)
+        assert code == _abi.AB_OK
+        _, _, final = srv.poll_sums(j)
+        srv.wait(k)
+        with pytest.raises(AlbedoError, match="AB_JOB_KEEP_SUMS"):
+            srv.poll_sums(k)
+    # Per-pixel counts never go down between snapshots and end at spp everywhere.
+    for a, b in zip(counts, counts[1:]):
+        assert np.all(b >= a)
+    assert np.all(final[..., 3] == 64)
+    # The final sums resolve on the host to the served image, bit for bit.
+    assert np.array_equal(albedo_amd.sums_resolve(final).view(np.uint32), img.view(np.uint32))
+    assert np.array_equal(img.view(np.uint32), room.render(*job[1:]).view(np.uint32))
