@@ -30,6 +30,7 @@ AB_TUNE_PLAIN, AB_TUNE_BATCH = 0, 1
 ESTIMATOR_PATH, ESTIMATOR_SCATTER_ONLY, ESTIMATOR_NORMALS = 0, 1, 2
 SCHED_FIFO, SCHED_SHARE, SCHED_SMALL_FIRST = 0, 1, 2
 JOB_KEEP_SUMS = 1
+BUILD_DEVICE_BLAS = 1
 SUMS_FRACTION_BITS = 28
 
 AB_MISS = 0xFFFFFFFF
@@ -83,7 +84,8 @@ class SceneDesc(ctypes.Structure):
     _fields_ = [("meshes", ctypes.POINTER(MeshDesc)), ("num_meshes", ctypes.c_uint32),
                 ("objects", ctypes.POINTER(ObjectDesc)), ("num_objects", ctypes.c_uint32),
                 ("materials", ctypes.POINTER(MaterialDesc)), ("num_materials", ctypes.c_uint32),
-                ("lights", ctypes.POINTER(LightDesc)), ("num_lights", ctypes.c_uint32)]
+                ("lights", ctypes.POINTER(LightDesc)), ("num_lights", ctypes.c_uint32),
+                ("build_flags", ctypes.c_uint32)]
 
 
 class Camera(ctypes.Structure):

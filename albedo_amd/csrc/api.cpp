@@ -1,11 +1,14 @@
 // C ABI: contexts, scenes and ray queries.
 
+#include "blas_build.hpp"
 #include "context.hpp"
 #include "kernels.hpp"
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <vector>
 
@@ -90,45 +93,86 @@ ab_status ab_scene_create(ab_context* ctx, const ab_scene_desc* desc, ab_scene**
     } catch (const std::bad_alloc&) {
         return fail(AB_OUT_OF_MEMORY, "ab_scene_create: out of host memory building the BVH");
     }
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hipFail(e, "hipSetDevice");
+    // Device-built BLASes (AB_BUILD_DEVICE_BLAS): sort and link on the device,
+    // then place their nodes after the host-built ones.
+    std::vector<std::unique_ptr<DeviceBlas>> dblas;
+    size_t device_nodes = 0;
+    for (const DeviceMesh& dm : hs.device_meshes) {
+        auto b = std::make_unique<DeviceBlas>();
+        if ((e = b->prepare(hs.tris.data() + dm.tri_base, dm.count, ctx->stream)) != hipSuccess)
+            return hipFail(e, "ab_scene_create: device BVH build");
+        const uint32_t root = uint32_t(hs.nodes.size() + device_nodes);
+        for (DevObject& o : hs.objects)
+            if (o.kind == kObjInstance && o.mesh == dm.mesh) o.blas_root = int32_t(root);
+        hs.max_blas_depth = std::max(hs.max_blas_depth, b->levels());
+        hs.blas_nodes += b->nodes();
+        device_nodes += b->nodes();
+        dblas.push_back(std::move(b));
+    }
+    if (hs.nodes.size() + device_nodes >= size_t(INT32_MAX))
+        return fail(AB_INVALID_ARGUMENT, "ab_scene_create: too many BVH nodes");
     // A traversal pushes one entry per interior node on its current root-to-leaf
     // path, so it needs at most (levels - 1) entries per tree; a BLAS traversal
-    // starts on top of the TLAS entries pending at the instance's leaf (traceRay),
-    // and the resumable query (RayQuery) keeps one more for the leaf's other objects.
-    const uint32_t path_stack =
+    // starts on top of the TLAS entries pending at the instance's leaf (the rest of
+    // that leaf waits in a register, not on the stack).
+    const uint32_t stack =
         (hs.tlas_depth ? hs.tlas_depth - 1 : 0) + (hs.max_blas_depth ? hs.max_blas_depth - 1 : 0);
-    const uint32_t stack = path_stack + 1;
     if (stack > kMaxStackDepth)
         return fail(AB_INVALID_ARGUMENT, "ab_scene_create: BVH too deep for the traversal stack");
 
     // One arena holds every array, each 256-byte aligned.
     struct Part {
         const void* src;
-        size_t bytes;
+        size_t bytes; // copied from src
         size_t off;
+        size_t room;  // bytes the part occupies (>= bytes)
     };
-    Part parts[] = {{hs.nodes.data(), hs.nodes.size() * sizeof(DevNode), 0},
-                    {hs.tris.data(), hs.tris.size() * sizeof(DevTri), 0},
-                    {hs.leafobj.data(), hs.leafobj.size() * sizeof(uint32_t), 0},
-                    {hs.objects.data(), hs.objects.size() * sizeof(DevObject), 0},
-                    {hs.spheres.data(), hs.spheres.size() * sizeof(double), 0},
-                    {hs.shading.data(), hs.shading.size() * sizeof(DevTriShading), 0},
-                    {hs.materials.data(), hs.materials.size() * sizeof(DevMaterial), 0},
-                    {hs.lights.data(), hs.lights.size() * sizeof(DevLight), 0}};
+    Part parts[] = {{hs.nodes.data(), hs.nodes.size() * sizeof(DevNode), 0,
+                     (hs.nodes.size() + device_nodes) * sizeof(DevNode)},
+                    {hs.tris.data(), hs.tris.size() * sizeof(DevTri), 0, 0},
+                    {hs.leafobj.data(), hs.leafobj.size() * sizeof(uint32_t), 0, 0},
+                    {hs.objects.data(), hs.objects.size() * sizeof(DevObject), 0, 0},
+                    {hs.spheres.data(), hs.spheres.size() * sizeof(double), 0, 0},
+                    {hs.shading.data(), hs.shading.size() * sizeof(DevTriShading), 0, 0},
+                    {hs.materials.data(), hs.materials.size() * sizeof(DevMaterial), 0, 0},
+                    {hs.lights.data(), hs.lights.size() * sizeof(DevLight), 0, 0}};
     size_t total = 0;
     for (Part& p : parts) {
+        p.room = std::max(p.room, p.bytes);
         p.off = total;
-        total += align256(std::max<size_t>(p.bytes, 16));
+        total += align256(std::max<size_t>(p.room, 16));
     }
-    hipError_t e = hipSetDevice(ctx->device);
-    if (e != hipSuccess) return hipFail(e, "hipSetDevice");
     void* arena = nullptr;
     if ((e = hipMalloc(&arena, total)) != hipSuccess) return hipFail(e, "hipMalloc(scene)");
-    std::vector<unsigned char> staging(total, 0);
-    for (const Part& p : parts)
-        if (p.bytes) std::memcpy(staging.data() + p.off, p.src, p.bytes);
-    if ((e = hipMemcpy(arena, staging.data(), total, hipMemcpyHostToDevice)) != hipSuccess) {
-        (void)hipFree(arena);
-        return hipFail(e, "hipMemcpy(scene)");
+    // Each array straight from its host vector (no staging copy); padding between
+    // parts and the device-built nodes' room are never read before written.
+    for (const Part& p : parts) {
+        if (!p.bytes) continue;
+        if ((e = hipMemcpy(static_cast<unsigned char*>(arena) + p.off, p.src, p.bytes, hipMemcpyHostToDevice)) !=
+            hipSuccess) {
+            (void)hipFree(arena);
+            return hipFail(e, "hipMemcpy(scene)");
+        }
+    }
+    {
+        // Device-built BLASes: nodes after the host's, triangles over their
+        // kept-order copies, in leaf order.
+        auto* nodes = reinterpret_cast<DevNode*>(static_cast<unsigned char*>(arena) + parts[0].off);
+        auto* tris = reinterpret_cast<DevTri*>(static_cast<unsigned char*>(arena) + parts[1].off);
+        size_t base = hs.nodes.size();
+        for (size_t i = 0; i < dblas.size() && e == hipSuccess; ++i) {
+            const DeviceMesh& dm = hs.device_meshes[i];
+            e = dblas[i]->emit(nodes + base, uint32_t(base), tris + dm.tri_base, dm.tri_base, ctx->stream);
+            base += dblas[i]->nodes();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        dblas.clear();
+        if (e != hipSuccess) {
+            (void)hipFree(arena);
+            return hipFail(e, "ab_scene_create: device BVH emission");
+        }
     }
     ab_scene* s = new (std::nothrow) ab_scene;
     if (!s) {
@@ -152,7 +196,6 @@ ab_status ab_scene_create(ab_context* ctx, const ab_scene_desc* desc, ab_scene**
     s->dev.num_lights = uint32_t(hs.lights.size());
     s->dev.sky_light = hs.sky_light;
     s->dev.stack_depth = stack;
-    s->dev.path_stack_depth = path_stack;
     s->dev.has_spheres = 0;
     for (const DevObject& o : hs.objects) s->dev.has_spheres |= o.kind == kObjSphere ? 1u : 0u;
     s->tlas_depth = hs.tlas_depth;

@@ -286,7 +286,7 @@ ab_status Engine::renderBatch(const std::vector<JobSpec>& specs, const std::vect
         const ab_status st = prepareJob(specs[i], jobs_[i]);
         if (st != AB_OK) return st;
         total += jobs_[i].work_count;
-        stack = std::max(stack, jobs_[i].scene.path_stack_depth);
+        stack = std::max(stack, jobs_[i].scene.stack_depth);
         bufs[i] = jobBuffers(jobs_[i]);
         accum_bytes += bufs[i].total;
     }
@@ -381,12 +381,12 @@ ab_status Engine::appendJob(const DevJob& prepared, unsigned long long* accum, u
                             uint32_t* index)
 {
     if (tableFull()) return fail(AB_REJECTED_BUDGET, "job table full");
-    if (prepared.scene.path_stack_depth > 64)
+    if (prepared.scene.stack_depth > 64)
         return fail(AB_INVALID_ARGUMENT, "scene BVH too deep for the wavefront engine (stack > 64)");
     // The bounce kernels follow the deepest scene in the table; queues do not
     // depend on them, so they can change between bounces.
-    if (stackClass(prepared.scene.path_stack_depth) > shape_.stack) {
-        const ab_status st = setStreamStack(prepared.scene.path_stack_depth);
+    if (stackClass(prepared.scene.stack_depth) > shape_.stack) {
+        const ab_status st = setStreamStack(prepared.scene.stack_depth);
         if (st != AB_OK) return st;
     }
     DevJob dj = prepared;

@@ -110,9 +110,8 @@ struct DevScene {
     int32_t tlas_root;
     uint32_t num_lights;
     int32_t sky_light;
-    uint32_t stack_depth;      // most traversal-stack entries a resumable query (RayQuery) holds
-    uint32_t path_stack_depth; // most entries traceRay holds (the path tracer's bounce kernels)
-    uint32_t has_spheres;      // any object is a sphere (else ray queries skip the FP64 test)
+    uint32_t stack_depth; // most traversal-stack entries a query holds (traceRay and RayQuery)
+    uint32_t has_spheres; // any object is a sphere (else ray queries skip the FP64 test)
 };
 
 } // namespace ab

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <thread>
 
 namespace ab {
 
@@ -319,6 +320,7 @@ uint64_t hashSceneDesc(const ab_scene_desc& d)
     if (d.num_materials) hs.bytes(d.materials, sizeof(ab_material_desc) * d.num_materials);
     hs.mix(d.num_lights);
     if (d.num_lights) hs.bytes(d.lights, sizeof(ab_light_desc) * d.num_lights);
+    hs.mix(d.build_flags);
     return hs.h;
 }
 
@@ -328,6 +330,10 @@ ab_status buildHostScene(const ab_scene_desc& d, HostScene& out, std::string& er
     if ((d.num_meshes && !d.meshes) || (d.num_objects && !d.objects) ||
         (d.num_materials && !d.materials) || (d.num_lights && !d.lights)) {
         err = "scene description has a null array with a non-zero count";
+        return AB_INVALID_ARGUMENT;
+    }
+    if (d.build_flags & ~AB_BUILD_DEVICE_BLAS) {
+        err = "unknown scene build flags " + std::to_string(d.build_flags);
         return AB_INVALID_ARGUMENT;
     }
     // ------------------------------------------------------------ materials
@@ -450,49 +456,92 @@ ab_status buildHostScene(const ab_scene_desc& d, HostScene& out, std::string& er
         MeshBuild& b = mb[mi];
         b.smooth = m.smooth != 0;
         b.shading_base = uint32_t(out.shading.size());
-        std::vector<Item> items(kept.size());
-        for (uint32_t t = 0; t < kept.size(); ++t) {
-            const uint32_t* ix = m.indices + 3 * kept[t];
-            Item& it = items[t];
-            double p[3][3];
-            for (int c = 0; c < 3; ++c) {
-                P(ix[c], p[c]);
-                it.box.grow(p[c]);
-            }
-            for (int k = 0; k < 3; ++k) it.c[k] = it.box.lo[k] / 2 + it.box.hi[k] / 2;
-            it.id = t;
-            b.box.grow(it.box);
-            // Shading record: face normal, corner normals, uv.
-            DevTriShading sh{};
-            double e1[3], e2[3], fn[3];
-            for (int k = 0; k < 3; ++k) {
-                e1[k] = p[1][k] - p[0][k];
-                e2[k] = p[2][k] - p[0][k];
-            }
-            cross3(e1, e2, fn);
-            for (int k = 0; k < 3; ++k) {
-                sh.p0[k] = m.positions[3 * ix[0] + k];
-                sh.p1[k] = m.positions[3 * ix[1] + k];
-                sh.p2[k] = m.positions[3 * ix[2] + k];
-            }
-            const double fl = std::sqrt(dot3(fn, fn));
-            for (int k = 0; k < 3; ++k) sh.fn[k] = float(fn[k] / fl);
-            float* cn[3] = {sh.n0, sh.n1, sh.n2};
-            for (int c = 0; c < 3; ++c) {
-                if (m.smooth) {
-                    const auto& s = vsum[ix[c]];
-                    const double l2 = s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
-                    if (std::isnormal(l2)) {
-                        const double l = std::sqrt(l2);
-                        for (int k = 0; k < 3; ++k) cn[c][k] = float(s[k] / l);
-                        continue;
-                    }
+        const bool on_device = (d.build_flags & AB_BUILD_DEVICE_BLAS) && kept.size() > 4;
+        // Unit vertex normals of a smooth mesh, once per vertex (none where the
+        // sum has no length: those corners take the face normal).
+        std::vector<std::array<float, 3>> vn(m.smooth ? m.num_vertices : 0);
+        std::vector<uint8_t> vn_ok(m.smooth ? m.num_vertices : 0, 0);
+        for (uint32_t v = 0; v < vn.size(); ++v) {
+            const auto& s = vsum[v];
+            const double l2 = s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
+            if (!std::isnormal(l2)) continue;
+            const double l = std::sqrt(l2);
+            for (int k = 0; k < 3; ++k) vn[v][k] = float(s[k] / l);
+            vn_ok[v] = 1;
+        }
+        // Per triangle, independently: its box (the SAH input) and its shading
+        // record (face normal, corner normals, uv); large meshes across threads.
+        const uint32_t nt = uint32_t(kept.size());
+        std::vector<Item> items(on_device ? 0 : nt);
+        out.shading.resize(size_t(b.shading_base) + nt);
+        const unsigned threads =
+            nt < (1u << 16) ? 1u : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<BBox> part_box(threads);
+        auto work = [&](unsigned w) {
+            const uint32_t t0 = uint32_t(uint64_t(nt) * w / threads), t1 = uint32_t(uint64_t(nt) * (w + 1) / threads);
+            BBox mine;
+            for (uint32_t t = t0; t < t1; ++t) {
+                const uint32_t* ix = m.indices + 3 * kept[t];
+                double p[3][3];
+                BBox tb;
+                for (int c = 0; c < 3; ++c) {
+                    P(ix[c], p[c]);
+                    tb.grow(p[c]);
                 }
-                for (int k = 0; k < 3; ++k) cn[c][k] = sh.fn[k];
+                mine.grow(tb);
+                if (!on_device) {
+                    Item& it = items[t];
+                    it.box = tb;
+                    for (int k = 0; k < 3; ++k) it.c[k] = tb.lo[k] / 2 + tb.hi[k] / 2;
+                    it.id = t;
+                }
+                DevTriShading sh{};
+                double e1[3], e2[3], fn[3];
+                for (int k = 0; k < 3; ++k) {
+                    e1[k] = p[1][k] - p[0][k];
+                    e2[k] = p[2][k] - p[0][k];
+                }
+                cross3(e1, e2, fn);
+                for (int k = 0; k < 3; ++k) {
+                    sh.p0[k] = m.positions[3 * ix[0] + k];
+                    sh.p1[k] = m.positions[3 * ix[1] + k];
+                    sh.p2[k] = m.positions[3 * ix[2] + k];
+                }
+                const double fl = std::sqrt(dot3(fn, fn));
+                for (int k = 0; k < 3; ++k) sh.fn[k] = float(fn[k] / fl);
+                float* cn[3] = {sh.n0, sh.n1, sh.n2};
+                for (int c = 0; c < 3; ++c)
+                    for (int k = 0; k < 3; ++k) cn[c][k] = m.smooth && vn_ok[ix[c]] ? vn[ix[c]][k] : sh.fn[k];
+                static const float kDefaultUv[6] = {0, 0, 1, 0, 1, 1};
+                for (int k = 0; k < 6; ++k) sh.uv[k] = m.uv ? m.uv[6 * kept[t] + k] : kDefaultUv[k];
+                out.shading[size_t(b.shading_base) + t] = sh;
             }
-            static const float kDefaultUv[6] = {0, 0, 1, 0, 1, 1};
-            for (int k = 0; k < 6; ++k) sh.uv[k] = m.uv ? m.uv[6 * kept[t] + k] : kDefaultUv[k];
-            out.shading.push_back(sh);
+            part_box[w] = mine;
+        };
+        if (threads == 1) {
+            work(0);
+        } else {
+            std::vector<std::thread> pool;
+            for (unsigned w = 0; w < threads; ++w) pool.emplace_back(work, w);
+            for (std::thread& th : pool) th.join();
+        }
+        for (const BBox& pb : part_box) b.box.grow(pb);
+        if (on_device) {
+            // Triangles in kept order; the device build sorts and places them.
+            out.tris.reserve(out.tris.size() + nt);
+            out.device_meshes.push_back({mi, uint32_t(out.tris.size()), uint32_t(kept.size())});
+            for (uint32_t t = 0; t < kept.size(); ++t) {
+                const uint32_t* ix = m.indices + 3 * kept[t];
+                DevTri dt{};
+                for (int k = 0; k < 3; ++k) {
+                    dt.p0[k] = m.positions[3 * ix[0] + k];
+                    dt.p1[k] = m.positions[3 * ix[1] + k];
+                    dt.p2[k] = m.positions[3 * ix[2] + k];
+                }
+                dt.prim = t;
+                out.tris.push_back(dt);
+            }
+            continue;
         }
         SahBuilder builder;
         const int32_t root = builder.build(items);

@@ -11,6 +11,14 @@
 
 namespace ab {
 
+// A mesh whose BLAS the device builds (AB_BUILD_DEVICE_BLAS): its kept
+// triangles sit in HostScene::tris at tri_base in kept order until then.
+struct DeviceMesh {
+    uint32_t mesh;
+    uint32_t tri_base;
+    uint32_t count;
+};
+
 // Everything the device needs for one scene, in host memory.
 struct HostScene {
     std::vector<DevNode> nodes;
@@ -28,6 +36,9 @@ struct HostScene {
     uint64_t tlas_nodes{0};
     uint64_t blas_nodes{0};
     uint64_t hash{0};
+    // Meshes left to the device build; their instances' blas_root is kNoChild
+    // until the build places their nodes, and max_blas_depth excludes them.
+    std::vector<DeviceMesh> device_meshes;
 };
 
 // Validates `desc` as the reference constructors do and builds the scene.

@@ -426,7 +426,7 @@ ab_status ab_job_submit(ab_context* ctx, const ab_job_desc* job, uint64_t* job_i
         return fail(AB_INVALID_ARGUMENT, "ab_job_submit: scene belongs to another context");
     const ab_status st = prepareJob(j->spec, j->prepared);
     if (st != AB_OK) return st;
-    if (j->prepared.scene.path_stack_depth > 64)
+    if (j->prepared.scene.stack_depth > 64)
         return fail(AB_INVALID_ARGUMENT, "scene BVH too deep for the wavefront engine (stack > 64)");
     if (job->flags & ~AB_JOB_KEEP_SUMS)
         return fail(AB_INVALID_ARGUMENT, "ab_job_submit: unknown job flags " + std::to_string(job->flags));

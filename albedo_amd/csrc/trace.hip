@@ -39,7 +39,7 @@ template <bool SPHERES>
 constexpr int queryWaves() { return SPHERES ? 6 : 7; }
 
 // One kernel for closest and any hit: `per_lane` rays per lane of each wave's slice.
-template <int STACK, bool ANY, bool SPHERES>
+template <int STACK, bool ANY, bool SPHERES, bool LEAF_IN_REG>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(queryWaves<SPHERES>(), 8))) void kQuery(
     DevScene sc, const float4* __restrict__ rays, uint4* __restrict__ hits, uint8_t* __restrict__ occluded, uint64_t n,
     uint32_t per_lane)
@@ -50,7 +50,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(queryWav
     uint64_t next = uint64_t(xcdRemap(blockIdx.x, gridDim.x)) * slice;
     const uint64_t end = next + slice < n ? next + slice : n;
     const LdsStack<kBlock> stk{stack + threadIdx.x};
-    RayQuery<kBlock, ANY, SPHERES> q;
+    RayQuery<kBlock, ANY, SPHERES, LEAF_IN_REG> q;
     constexpr uint64_t kNone = ~0ull;
     uint64_t id = kNone;
     while (true) {
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(queryWav
     }
 }
 
-template <int STACK, bool ANY>
+template <int STACK, bool ANY, bool LEAF_IN_REG>
 hipError_t launchQ(const DevScene& sc, const ab_ray* rays, ab_hit* hits, uint8_t* occ, uint64_t n, hipStream_t s)
 {
     const uint64_t per_lane = (n + 64ull * kTargetWaves - 1) / (64ull * kTargetWaves);
@@ -89,12 +89,21 @@ hipError_t launchQ(const DevScene& sc, const ab_ray* rays, ab_hit* hits, uint8_t
     const float4* r = reinterpret_cast<const float4*>(rays);
     uint4* h = reinterpret_cast<uint4*>(hits);
     if (sc.has_spheres)
-        hipLaunchKernelGGL((kQuery<STACK, ANY, true>), dim3(uint32_t(waves)), dim3(kBlock), 0, s, sc, r, h, occ, n,
-                           uint32_t(per_lane));
+        hipLaunchKernelGGL((kQuery<STACK, ANY, true, LEAF_IN_REG>), dim3(uint32_t(waves)), dim3(kBlock), 0, s, sc, r,
+                           h, occ, n, uint32_t(per_lane));
     else
-        hipLaunchKernelGGL((kQuery<STACK, ANY, false>), dim3(uint32_t(waves)), dim3(kBlock), 0, s, sc, r, h, occ, n,
-                           uint32_t(per_lane));
+        hipLaunchKernelGGL((kQuery<STACK, ANY, false, LEAF_IN_REG>), dim3(uint32_t(waves)), dim3(kBlock), 0, s, sc, r,
+                           h, occ, n, uint32_t(per_lane));
     return hipGetLastError();
+}
+
+// The query keeps the rest of a TLAS leaf on the stack when that extra entry
+// stays within the scene's stack class (cheaper), else in a register.
+template <int STACK, bool ANY>
+hipError_t launchClass(const DevScene& sc, const ab_ray* rays, ab_hit* hits, uint8_t* occ, uint64_t n, hipStream_t s)
+{
+    if (sc.stack_depth + 1 <= uint32_t(STACK)) return launchQ<STACK, ANY, false>(sc, rays, hits, occ, n, s);
+    return launchQ<STACK, ANY, true>(sc, rays, hits, occ, n, s);
 }
 
 template <bool ANY>
@@ -102,11 +111,11 @@ hipError_t launchByStack(const DevScene& sc, const ab_ray* rays, ab_hit* hits, u
 {
     if (n == 0) return hipSuccess;
     if (n > (uint64_t(1) << 31) * 64) return hipErrorInvalidValue;
-    if (sc.stack_depth <= kShortStack) return launchQ<kShortStack, ANY>(sc, rays, hits, occ, n, s);
-    if (sc.stack_depth <= 32) return launchQ<32, ANY>(sc, rays, hits, occ, n, s);
-    if (sc.stack_depth <= 64) return launchQ<64, ANY>(sc, rays, hits, occ, n, s);
-    if (sc.stack_depth <= 128) return launchQ<128, ANY>(sc, rays, hits, occ, n, s);
-    if (sc.stack_depth <= kMaxStackDepth) return launchQ<256, ANY>(sc, rays, hits, occ, n, s);
+    if (sc.stack_depth <= kShortStack) return launchClass<kShortStack, ANY>(sc, rays, hits, occ, n, s);
+    if (sc.stack_depth <= 32) return launchClass<32, ANY>(sc, rays, hits, occ, n, s);
+    if (sc.stack_depth <= 64) return launchClass<64, ANY>(sc, rays, hits, occ, n, s);
+    if (sc.stack_depth <= 128) return launchClass<128, ANY>(sc, rays, hits, occ, n, s);
+    if (sc.stack_depth <= kMaxStackDepth) return launchClass<256, ANY>(sc, rays, hits, occ, n, s);
     return hipErrorInvalidValue;
 }
 

@@ -391,21 +391,24 @@ __device__ __forceinline__ bool traceRay(const DevScene& sc, const RayF& r, HitF
 // the other lanes of its wave continue (see kernels that refill lanes). Visits
 // the same nodes and primitives with the same culling as traceRay, so results
 // are identical. A TLAS leaf is taken one object at a time; the leaf's other
-// objects wait on the stack as a shorter leaf reference, so the stack needs one
-// entry more than traceRay's. SPHERES = false compiles the FP64 sphere test out,
-// for scenes whose objects are all mesh instances.
+// objects wait as a shorter leaf reference, either on the stack (one entry more
+// than traceRay's) or, with LEAF_IN_REG, in a register (pending) taken up before
+// the next TLAS stack entry (no extra entry, one more live register and check).
+// SPHERES = false compiles the FP64 sphere test out, for scenes whose objects are
+// all mesh instances.
 // Triangles a step tests: with one, a step costs about the same at a leaf as at
 // a node, and lanes of a wave at different kinds of visit waste least (measured
 // on the trefoil sets: 6 % faster than whole leaves of up to 4).
 constexpr uint32_t kLeafTrisPerStep = 1;
 
-template <int BLOCK, bool ANY, bool SPHERES = true>
+template <int BLOCK, bool ANY, bool SPHERES = true, bool LEAF_IN_REG = false>
 struct RayQuery {
     RayF r;          // world-space ray
     BoxRay box;      // box-test form of the ray of the current level (world or instance space)
     ShearRay sh;     // triangle-test form of the instance-space ray
     HitF best;
     int32_t ref;
+    int32_t pending; // the current TLAS leaf's objects not yet taken, or kNoChild
     int sp, sp_blas; // stack depth, and where the current instance's entries start
     uint32_t obj;    // instance being traversed
     bool in_blas, done;
@@ -419,6 +422,7 @@ struct RayQuery {
         best.b1 = best.b2 = 0.0f;
         box = makeBoxRay(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz);
         ref = sc.tlas_root;
+        pending = kNoChild;
         sp = sp_blas = 0;
         in_blas = false;
         done = sc.tlas_root == kNoChild;
@@ -433,6 +437,12 @@ struct RayQuery {
             // Back in the TLAS: the world-space box ray again.
             in_blas = false;
             box = makeBoxRay(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz);
+        }
+        if (LEAF_IN_REG && !in_blas && pending != kNoChild) {
+            // The rest of the TLAS leaf comes before the next stack entry.
+            ref = pending;
+            pending = kNoChild;
+            return;
         }
         if (sp == 0) {
             done = true;
@@ -508,7 +518,10 @@ struct RayQuery {
             return;
         }
         const uint32_t o = sc.leafobj[first];
-        if (count > 1) stk.put(sp++, encodeLeaf(first + 1, count - 1));
+        if (count > 1) {
+            if (LEAF_IN_REG) pending = encodeLeaf(first + 1, count - 1);
+            else stk.put(sp++, encodeLeaf(first + 1, count - 1));
+        }
         const DevObject& d = sc.objects[o];
         if (SPHERES && d.kind == kObjSphere) {
             double t;

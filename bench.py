@@ -124,6 +124,53 @@ def batch_leg(ctx, n_jobs=16):
             "speedup": round(plain / batched, 3), "tuned_plain": cfg_plain, "tuned_batched": cfg_batch}
 
 
+def build_leg(ctx):
+    """BVH build of the serving scene (1,310,720-triangle displaced icosphere x 64
+    instances): scene creation with the host SAH BLAS vs the device linear BVH
+    (AB_BUILD_DEVICE_BLAS), and closest-hit throughput of each on 2^22 rays into
+    the instance grid (hits are identical; the tree quality differs)."""
+    import torch
+    from albedo_amd import workloads as W
+    s = W.icosphere_scene()
+    out = {"workload": "icosphere 1,310,720 triangles x 64 instances; 2^22 rays from the serving camera"}
+    rng = np.random.default_rng(21)
+    n = 1 << 22
+    org = np.tile(np.array(W.ICOSPHERE_CAMERA.look_from, np.float32), (n, 1))
+    tgt = np.stack([rng.uniform(-6, 6, n), rng.uniform(0.0, 1.5, n), rng.uniform(-6, 6, n)], 1)
+    d = (tgt - org).astype(np.float64)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = org
+    rays[:, 4:7] = d
+    rays[:, 7] = np.inf
+    d_rays = torch.from_numpy(rays).cuda()
+    d_hits = torch.zeros((n, 4), dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+    digests = {}
+    for name, device in (("host_sah", False), ("device_lbvh", True)):
+        t0 = time.perf_counter()
+        g = ctx.scene(s, device_blas=device)
+        build_s = time.perf_counter() - t0
+        g.trace_closest_device(d_rays.data_ptr(), d_hits.data_ptr(), n, stream.cuda_stream)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ms = []
+        for _ in range(3):
+            e0.record(stream)
+            g.trace_closest_device(d_rays.data_ptr(), d_hits.data_ptr(), n, stream.cuda_stream)
+            e1.record(stream)
+            e1.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        digests[name] = hash(d_hits.cpu().numpy().tobytes())
+        info = g.info()
+        out[name] = {"scene_create_ms": round(build_s * 1e3, 1), "blas_nodes": info["blas_nodes"],
+                     "stack_depth": info["stack_depth"], "mrays_s": round(n / (min(ms) * 1e-3) / 1e6, 1)}
+        g.release()
+        print("build: %s done" % name, file=sys.stderr, flush=True)
+    out["hits_identical"] = digests["host_sah"] == digests["device_lbvh"]
+    return out
+
+
 def _fifo_latencies(arrivals, service):
     """Latencies of a one-job-at-a-time server taking jobs in arrival order."""
     free, out = 0.0, []
@@ -220,8 +267,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--trace-only", action="store_true", help="closest-hit benchmark only")
-    ap.add_argument("--legs", default="pt,batch,serve",
-                    help="legs after the closest-hit benchmark: any of pt, batch, serve (comma separated)")
+    ap.add_argument("--legs", default="pt,batch,serve,build",
+                    help="legs after the closest-hit benchmark: any of pt, batch, serve, build (comma separated)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy, the reference for counter-based traffic")
     args = ap.parse_args()
@@ -318,6 +365,8 @@ def main():
         out["batched"] = batch_leg(ctx)
     if "serve" in legs:
         out["served"] = serve_leg(ctx)
+    if "build" in legs:
+        out["bvh_build"] = build_leg(ctx)
     print(json.dumps(out))
 
 
