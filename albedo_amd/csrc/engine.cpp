@@ -210,6 +210,12 @@ ab_status Engine::readCounters(Counters& out)
             "hipMemcpy(counters)");
     HIP_TRY(hipStreamSynchronize(stream_), "hipStreamSynchronize");
     out = *h_counters_;
+    // The last bounce's terminations and shadow rays, which the device adds at
+    // the next bounce, counted now.
+    out.paths_done += out.n_term;
+    out.shadow_rays += out.n_shadow;
+    out.n_free += out.n_term;
+    out.n_term = out.n_shadow = 0;
     return AB_OK;
 }
 

@@ -42,6 +42,22 @@ __device__ __forceinline__ float powerHeuristic(float chosen, float other)
     return t > 0.0f ? c2 / t : 0.0f;
 }
 
+// The generation a bounce starts with, from the counters as the previous bounce
+// left them: the slots its finish freed (n_term, pushed above n_free) are free
+// again, and new samples fill free slots up to the work exposed. kGenerate and
+// kExtend compute it; kPlanBins applies it to the counters after both.
+struct Generation {
+    uint32_t free_top; // free slots
+    uint32_t n_new;    // samples generated
+};
+
+__device__ __forceinline__ Generation pendingGeneration(const Counters* c)
+{
+    const uint32_t free_top = c->n_free + c->n_term;
+    const unsigned long long left = c->work_total - c->work_next;
+    return {free_top, uint32_t(left < free_top ? left : free_top)};
+}
+
 __device__ __forceinline__ uint32_t gridStart() { return blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ __forceinline__ uint32_t gridStride() { return gridDim.x * blockDim.x; }
 
@@ -56,7 +72,8 @@ __device__ __forceinline__ uint32_t gridStride() { return gridDim.x * blockDim.x
 __global__ void kGenerate(PathState ps)
 {
     const Counters* c = ps.ctr;
-    const uint32_t n_new = c->n_new, free_top = c->n_free, act0 = c->n_active;
+    const Generation g = pendingGeneration(c);
+    const uint32_t n_new = g.n_new, free_top = g.free_top, act0 = c->n_active;
     const unsigned long long w0 = c->work_next;
     const uint32_t seg_lo = c->seg_lo, seg_hi = c->seg_hi;
     for (uint32_t i = gridStart(); i < n_new; i += gridStride()) {
@@ -99,16 +116,6 @@ __global__ void kGenerate(PathState ps)
     }
 }
 
-__global__ void kPostGenerate(PathState ps)
-{
-    Counters* c = ps.ctr;
-    const uint32_t n = c->n_new;
-    c->n_active += n;
-    c->n_free -= n;
-    c->work_next += n;
-    c->camera_rays += n;
-    c->n_new = 0;
-}
 
 // ------------------------------------------------------ queue construction
 //
@@ -259,7 +266,8 @@ template <int BLOCK, int STACK>
 __global__ __launch_bounds__(BLOCK) void kExtend(PathState ps)
 {
     __shared__ int32_t stack[STACK * BLOCK];
-    const uint32_t n = ps.ctr->n_active;
+    // The queue holds the surviving paths and, after them, this bounce's new ones.
+    const uint32_t n = ps.ctr->n_active + pendingGeneration(ps.ctr).n_new;
     WaveCounts<kBins> wc;
     for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
         const uint32_t i = base + threadIdx.x;
@@ -287,7 +295,19 @@ __global__ __launch_bounds__(1024) void kPlanBins(PathState ps, uint32_t grid)
     __shared__ uint32_t totals[kBins];
     scanBlockCounts<kBins>(ps.block_count, ps.block_off, grid, true, totals);
     if (threadIdx.x == 0) {
+        // The bookkeeping of the last finish and of this bounce's generation, both
+        // already used by kGenerate and kExtend (pendingGeneration) and applied
+        // here, where one thread runs after them anyway.
         Counters* c = ps.ctr;
+        const Generation g = pendingGeneration(c);
+        c->paths_done += c->n_term;
+        c->shadow_rays += c->n_shadow;
+        c->n_term = 0;
+        c->n_shadow = 0;
+        c->n_free = g.free_top - g.n_new;
+        c->n_active += g.n_new;
+        c->work_next += g.n_new;
+        c->camera_rays += g.n_new;
         c->extend_rays += c->n_active;
         c->n_in = c->n_active;
         for (int b = 0; b < kBins; ++b) c->bin_count[b] = totals[b];
@@ -803,16 +823,6 @@ __global__ void kResolve(const unsigned long long* accum, uint64_t npx, float* m
     }
 }
 
-__global__ void kPost(PathState ps)
-{
-    Counters* c = ps.ctr;
-    c->n_free += c->n_term;
-    c->paths_done += c->n_term;
-    c->shadow_rays += c->n_shadow;
-    const unsigned long long left = c->work_total - c->work_next;
-    c->n_new = uint32_t(left < c->n_free ? left : c->n_free);
-}
-
 __global__ void kReset(PathState ps, unsigned long long total, uint32_t n_segs)
 {
     const uint32_t i0 = gridStart();
@@ -824,7 +834,6 @@ __global__ void kReset(PathState ps, unsigned long long total, uint32_t n_segs)
         c->work_total = total;
         c->seg_lo = 0;
         c->seg_hi = n_segs;
-        c->n_new = uint32_t(total < ps.capacity ? total : ps.capacity);
     }
 }
 
@@ -834,15 +843,12 @@ __global__ void kAddWork(PathState ps, unsigned long long total, uint32_t seg_lo
     c->work_total = total;
     c->seg_lo = seg_lo;
     c->seg_hi = seg_hi;
-    const unsigned long long left = total - c->work_next;
-    c->n_new = uint32_t(left < c->n_free ? left : c->n_free);
 }
 
 template <int BLOCK, int STACK>
 hipError_t bounce(const PathState& ps, uint32_t grid, hipStream_t s)
 {
     hipLaunchKernelGGL(kGenerate, dim3(grid), dim3(BLOCK), 0, s, ps);
-    hipLaunchKernelGGL(kPostGenerate, dim3(1), dim3(1), 0, s, ps);
     hipLaunchKernelGGL((kExtend<BLOCK, STACK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     hipLaunchKernelGGL(kPlanBins, dim3(1), dim3(1024), 0, s, ps, grid);
     hipLaunchKernelGGL((kScatterBins<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
@@ -852,7 +858,6 @@ hipError_t bounce(const PathState& ps, uint32_t grid, hipStream_t s)
     hipLaunchKernelGGL((kScatterShade<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     hipLaunchKernelGGL((kShadow<BLOCK, STACK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     hipLaunchKernelGGL((kFinish<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
-    hipLaunchKernelGGL(kPost, dim3(1), dim3(1), 0, s, ps);
     return hipGetLastError();
 }
 

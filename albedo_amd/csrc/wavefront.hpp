@@ -50,13 +50,16 @@ struct alignas(8) DevSegment {
 };
 
 // Counters shared by the kernels of one engine.
+// n_term and n_shadow of a bounce are added to paths_done and shadow_rays (and
+// the terminated paths' slots to n_free) at the next bounce's kPlanBins; the
+// host adds what is still pending when it reads the block (readCounters).
 struct alignas(16) Counters {
     uint32_t n_active;     // paths queued for extension
     uint32_t n_in;         // paths being shaded this bounce
     uint32_t n_shadow;
     uint32_t n_term;
     uint32_t n_free;
-    uint32_t n_new;        // paths generate() creates this round
+    uint32_t pad0;
     uint32_t bin_count[8]; // paths per material bin in the last bounce
     uint32_t seg_lo;       // live segments, logical ring indices [seg_lo, seg_hi)
     uint32_t seg_hi;
