@@ -261,10 +261,16 @@ ab_status Engine::runUntilDone(const LaunchShape& shape, bool use_graph, Counter
     // bounce generates up to `capacity` paths, so this many bounces always suffice;
     // running past it means the engine's bookkeeping is broken.
     uint64_t max_depth = 0;
-    for (const DevJob& j : jobs_) max_depth = std::max<uint64_t>(max_depth, j.max_depth);
+    uint64_t path_bounces = 1; // bounces a path can take (the normals view: one)
+    for (const DevJob& j : jobs_) {
+        max_depth = std::max<uint64_t>(max_depth, j.max_depth);
+        path_bounces = std::max<uint64_t>(path_bounces, j.estimator == AB_ESTIMATOR_NORMALS ? 1 : j.max_depth + 1);
+    }
     const uint64_t limit = 4 * (max_depth + 2) * (work_total_ / capacity_ + 2) + 64;
     uint64_t bounces = 0;
-    uint32_t pairs_per_check = 2;
+    // The first check comes when paths started in the first bounce can have
+    // finished (all of a render that fits the path capacity), at most 16 bounces in.
+    uint32_t pairs_per_check = uint32_t(std::min<uint64_t>((path_bounces + 2) / 2, 8));
     while (true) {
         if (bounces > limit) return fail(AB_DEVICE_ERROR, "render did not finish within its bounce bound");
         bounces += 2 * pairs_per_check;
