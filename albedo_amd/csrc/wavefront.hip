@@ -662,16 +662,35 @@ __global__ __launch_bounds__(BLOCK) void kCountShade(PathState ps)
     storeBlockCounts<BLOCK, 3>(wc, ps.block_count);
 }
 
-__global__ __launch_bounds__(1024) void kPlanShade(PathState ps, uint32_t grid)
+// Sums of block_count[b * 8 + q] over the blocks b < limit, for each q < NQ, to
+// every thread of the block. Each block of a scatter kernel sums the counts of
+// the blocks before it itself (at most kMaxGrid x NQ words, from L2), so no
+// one-block scan kernel has to run between the count and the scatter.
+template <int BLOCK, int NQ>
+__device__ __forceinline__ void countsBefore(const uint32_t* block_count, uint32_t limit, uint32_t (&sum)[NQ])
 {
-    __shared__ uint32_t totals[3];
-    scanBlockCounts<3>(ps.block_count, ps.block_off, grid, false, totals);
-    if (threadIdx.x == 0) {
-        Counters* c = ps.ctr;
-        c->n_active = totals[0];
-        c->n_shadow = totals[1];
-        c->n_term = totals[2];
+    __shared__ uint32_t s[BLOCK / 64][NQ];
+    uint32_t v[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) v[q] = 0;
+    for (uint32_t b = threadIdx.x; b < limit; b += BLOCK) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) v[q] += block_count[b * 8 + q];
     }
+    const uint32_t wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        for (int off = 32; off >= 1; off >>= 1) v[q] += __shfl_xor(v[q], off);
+        if (__lane_id() == 0) s[wave][q] = v[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        uint32_t t = 0;
+        for (uint32_t w = 0; w < uint32_t(BLOCK / 64); ++w) t += s[w][q];
+        sum[q] = t;
+    }
+    __syncthreads();
 }
 
 // Writes the shaded paths into the next extension queue and the shadow and
@@ -681,8 +700,19 @@ __global__ __launch_bounds__(BLOCK) void kScatterShade(PathState ps)
 {
     const uint32_t n = ps.ctr->n_in;
     uint32_t run[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) run[q] = ps.block_off[blockIdx.x * 8 + q];
+    countsBefore<BLOCK, 3>(ps.block_count, blockIdx.x, run);
+    if (blockIdx.x == 0) {
+        // The queue sizes for the kernels after this one (no block of this one
+        // reads them).
+        uint32_t all[3];
+        countsBefore<BLOCK, 3>(ps.block_count, gridDim.x, all);
+        if (threadIdx.x == 0) {
+            Counters* c = ps.ctr;
+            c->n_active = all[0];
+            c->n_shadow = all[1];
+            c->n_term = all[2];
+        }
+    }
     for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
         const uint32_t i = base + threadIdx.x;
         const uint32_t flags = i < n ? ps.key[i] : 0u;
@@ -854,7 +884,6 @@ hipError_t bounce(const PathState& ps, uint32_t grid, hipStream_t s)
     hipLaunchKernelGGL((kScatterBins<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     hipLaunchKernelGGL((kShade<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     hipLaunchKernelGGL((kCountShade<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
-    hipLaunchKernelGGL(kPlanShade, dim3(1), dim3(1024), 0, s, ps, grid);
     hipLaunchKernelGGL((kScatterShade<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     hipLaunchKernelGGL((kShadow<BLOCK, STACK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     hipLaunchKernelGGL((kFinish<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
