@@ -94,6 +94,13 @@ __global__ __launch_bounds__(kBlock) void kKeys(const Box* boxes, uint32_t n, co
     keys[i] = blas::mortonKey(boxes[i], cmin, cmax, bits, idx_bits, i);
 }
 
+__global__ __launch_bounds__(kBlock) void kSplitKeys(const unsigned long long* sorted, uint32_t n, int key_bits,
+                                                      int idx_bits, unsigned long long* out)
+{
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p < n) out[p] = blas::splitKey(sorted[p], p, key_bits, idx_bits, blas::kSplitTopBits);
+}
+
 __global__ __launch_bounds__(kBlock) void kHierarchy(const unsigned long long* keys, uint32_t n, int32_t* child,
                                                       uint32_t* parent, uint32_t* first, uint32_t* count)
 {
@@ -235,7 +242,9 @@ hipError_t DeviceBlas::prepare(const DevTri* host_tris, uint32_t n, hipStream_t 
     size_t bytes = scratch_bytes_;
     if ((e = hipcub::DeviceRadixSort::SortKeys(scratch_, bytes, keys_, sorted_, int(n), 0, key_bits, s)) != hipSuccess)
         return e;
-    hipLaunchKernelGGL(kHierarchy, dim3(blocksFor(m)), dim3(kBlock), 0, s, sorted_, n, child_, parent_, first_, count_);
+    // The unsorted keys are spent: their buffer takes the split keys.
+    hipLaunchKernelGGL(kSplitKeys, dim3(blocksFor(n)), dim3(kBlock), 0, s, sorted_, n, key_bits, idx_bits_, keys_);
+    hipLaunchKernelGGL(kHierarchy, dim3(blocksFor(m)), dim3(kBlock), 0, s, keys_, n, child_, parent_, first_, count_);
     hipLaunchKernelGGL(kBottomUp, dim3(blocksFor(n)), dim3(kBlock), 0, s, n, idx_bits_, sorted_, boxes, child_,
                        parent_, count_, levels_v_, visits_);
     hipLaunchKernelGGL(kKeep, dim3(blocksFor(m)), dim3(kBlock), 0, s, count_, uint32_t(m), keep_);

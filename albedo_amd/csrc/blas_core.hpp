@@ -66,6 +66,25 @@ AB_HD inline unsigned long long mortonKey(const Box& b, const float* cmin, const
     return (code << idx_bits) | i;
 }
 
+// The key the hierarchy is built over, for sorted position p: the top `top`
+// bits of the Morton code, then p. Still sorted and unique; the tree splits on
+// space for its top levels and halves position ranges below, which bounds its
+// depth near log2(n) where Morton codes alone give long chains (uneven surfaces).
+// Measured on the 1.3M-triangle displaced icosphere x 64 instances (2^22 rays):
+// Morton alone 29 levels, 722 Mrays/s; 15 bits 25 levels, 836 Mrays/s; 12 bits
+// 816; 6 bits 667; positions alone 20 levels, 475.
+constexpr int kSplitTopBits = 15;
+AB_HD inline unsigned long long splitKey(unsigned long long sorted_key, uint32_t p, int key_bits, int idx_bits,
+                                         int top)
+{
+    const int morton_bits = key_bits - idx_bits;
+    if (top > morton_bits) top = morton_bits;
+    const unsigned long long morton = sorted_key >> idx_bits;
+    const unsigned long long head = top > 0 ? morton >> (morton_bits - top) : 0ull;
+    return (head << 32) | p;
+}
+static_assert(kSplitTopBits >= 0 && kSplitTopBits <= 32, "split key: head bits above a 32-bit position");
+
 // Length of the common prefix of keys a and b; -1 when b is outside [0, n).
 AB_HD inline int prefix(const unsigned long long* keys, int n, int a, int b)
 {

@@ -88,14 +88,19 @@ def path_tracing_leg(ctx, cpu=True):
     return out
 
 
-def batch_leg(ctx, n_jobs=16):
-    """Mixed jobs (resolutions 256x144..960x540, 4..64 spp, room / full-scene
-    variants with 5..200 random spheres): one-at-a-time vs one batched launch."""
+def batch_leg(ctx, n_jobs=16, full=False):
+    """Mixed jobs (room / full-scene variants with 5..200 random spheres): one at a
+    time vs one batched launch, each with its autotuned configuration. Default:
+    16 jobs, 256x144..960x540, 4..64 spp; full: the 64-job mix of resolutions
+    256x144..1920x1080 and 4..256 spp."""
     import albedo_amd
     from albedo_amd import workloads as W
     from albedo_amd.scene import CameraSettings
     knot = W.trefoil_mesh()
-    mix = W.job_mix(n_jobs, resolutions=W.RESOLUTIONS[:3], spas=W.SPAS[:3], knot=knot)
+    if full:
+        mix = W.job_mix(n_jobs, knot=knot)
+    else:
+        mix = W.job_mix(n_jobs, resolutions=W.RESOLUTIONS[:3], spas=W.SPAS[:3], knot=knot)
     c = W.ROOM_CAMERA
     jobs = []
     for scene, (w, h), spa, depth, seed in mix:
@@ -117,7 +122,8 @@ def batch_leg(ctx, n_jobs=16):
     ctx.set_launch_config(base.block_size, base.paths_in_flight, base.grid_blocks, base.use_graph)
     rays_b = st["camera_rays"] + st["extension_rays"] + st["shadow_rays"]
     samples = sum(j[2] * j[3] * j[4] ** 2 for j in jobs)
-    return {"workload": "%d mixed jobs, 256x144..960x540, 4..64 spp, 5..200 random spheres" % n_jobs,
+    sizes = "256x144..1920x1080, 4..256 spp" if full else "256x144..960x540, 4..64 spp"
+    return {"workload": "%d mixed jobs, %s, 5..200 random spheres" % (n_jobs, sizes),
             "plain_s": round(plain, 3), "batched_s": round(batched, 3),
             "plain_mrays_s": round(rays_plain / plain / 1e6, 1), "batched_mrays_s": round(rays_b / batched / 1e6, 1),
             "batched_mpixel_spp_s": round(samples / batched / 1e6, 1),
@@ -268,7 +274,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--trace-only", action="store_true", help="closest-hit benchmark only")
     ap.add_argument("--legs", default="pt,batch,serve,build",
-                    help="legs after the closest-hit benchmark: any of pt, batch, serve, build (comma separated)")
+                    help="legs after the closest-hit benchmark: any of pt, batch, serve, build, and the "
+                         "longer batch64 (64 jobs up to 1920x1080 at 256 spp), comma separated")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy, the reference for counter-based traffic")
     args = ap.parse_args()
@@ -363,6 +370,8 @@ def main():
         out["path_tracing"] = path_tracing_leg(ctx, cpu=not args.no_cpu)
     if "batch" in legs:
         out["batched"] = batch_leg(ctx)
+    if "batch64" in legs:
+        out["batched_64"] = batch_leg(ctx, 64, full=True)
     if "serve" in legs:
         out["served"] = serve_leg(ctx)
     if "build" in legs:

@@ -148,12 +148,20 @@ int main(int argc, char** argv)
     std::sort(sorted.begin(), sorted.end());
     for (uint32_t i = 1; i < n; ++i) check(sorted[i] != sorted[i - 1], "unique keys", i);
     const unsigned long long mask = (1ull << idx_bits) - 1ull;
+    // The keys the hierarchy splits on (kSplitKeys): the Morton codes' top bits
+    // and the sorted position (blas::kSplitTopBits unless given; a negative
+    // value keeps the full sorted keys).
+    const int top = argc > 3 ? std::atoi(argv[3]) : blas::kSplitTopBits;
+    std::vector<unsigned long long> hkeys(sorted);
+    if (top >= 0)
+        for (uint32_t p = 0; p < n; ++p) hkeys[p] = blas::splitKey(sorted[p], p, 3 * bits + idx_bits, idx_bits, top);
+    for (uint32_t i = 1; i < n; ++i) check(hkeys[i] > hkeys[i - 1], "split keys increase", i);
 
     // 4. Hierarchy (kHierarchy).
     std::vector<int32_t> child(2 * size_t(m), INT32_MIN);
     std::vector<uint32_t> parent(2 * size_t(n) - 1, 0xdeadbeefu), first(m), count(m);
     for (uint32_t i = 0; i < m; ++i)
-        blas::hierarchyNode(sorted.data(), int(n), int(i), child.data(), parent.data(), first.data(), count.data());
+        blas::hierarchyNode(hkeys.data(), int(n), int(i), child.data(), parent.data(), first.data(), count.data());
     std::vector<int> refs(2 * size_t(n) - 1, 0);
     for (uint32_t i = 0; i < m; ++i)
         for (int c = 0; c < 2; ++c) {

@@ -20,13 +20,14 @@ def checker():
     return CHECK
 
 
-def run(checker, tmp_path, positions, faces, rays):
+def run(checker, tmp_path, positions, faces, rays, top=None):
     path = tmp_path / "mesh.bin"
     with open(path, "wb") as f:
         np.array([len(positions), len(faces)], np.uint32).tofile(f)
         np.asarray(positions, np.float32).tofile(f)
         np.asarray(faces, np.uint32).tofile(f)
-    r = subprocess.run([checker, str(path), str(rays)], capture_output=True, text=True, timeout=300)
+    args = [checker, str(path), str(rays)] + ([] if top is None else [str(top)])
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     words = r.stdout.split()
     return {words[i]: int(words[i + 1]) for i in range(1, len(words) - 1, 2)}
@@ -38,9 +39,11 @@ def test_trefoil(checker, tmp_path):
     assert out["n"] == 32768 and out["hits"] > 100
 
 
-def test_displaced_icosphere_level_5(checker, tmp_path):
+@pytest.mark.parametrize("top", [None, -1, 0, 6])
+def test_displaced_icosphere_level_5(checker, tmp_path, top):
+    # The device's split keys (None), plain Morton keys (-1), positions only (0).
     m = W.displaced_icosphere(level=5)
-    out = run(checker, tmp_path, m.positions, m.faces, 400)
+    out = run(checker, tmp_path, m.positions, m.faces, 400, top)
     assert out["n"] == 20480 and out["levels"] <= 32
 
 
@@ -48,7 +51,8 @@ def test_serving_mesh_full_size(checker, tmp_path):
     # The 1,310,720-triangle serving mesh: structure in full, 30 rays by brute force.
     m = W.displaced_icosphere(level=8)
     out = run(checker, tmp_path, m.positions, m.faces, 30)
-    assert out["n"] == 1310720 and out["levels"] < 40
+    # 25 levels keep the serving scene (TLAS 9 levels) within the 32-entry stack.
+    assert out["n"] == 1310720 and out["levels"] <= 25
 
 
 def test_many_triangles_with_one_centroid(checker, tmp_path):
