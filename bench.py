@@ -109,16 +109,20 @@ def batch_leg(ctx, n_jobs=16, full=False):
     base = ctx.launch_config()
     # Each form runs with the launch configuration the autotuner picks for it.
     cfg_plain, _ = ctx.autotune(jobs, batched=False)
+    print("batch: tuned the plain form", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     rays_plain = 0
     for j in jobs:
         _, st = j[0].render(*j[1:], stats=True)
         rays_plain += st["camera_rays"] + st["extension_rays"] + st["shadow_rays"]
     plain = time.perf_counter() - t0
+    print("batch: %d jobs one at a time in %.2f s" % (len(jobs), plain), file=sys.stderr, flush=True)
     cfg_batch, _ = ctx.autotune(jobs, batched=True)
+    print("batch: tuned the batched form", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     _, st = albedo_amd.render_batch(ctx, jobs)
     batched = time.perf_counter() - t0
+    print("batch: one batched launch in %.2f s" % batched, file=sys.stderr, flush=True)
     ctx.set_launch_config(base.block_size, base.paths_in_flight, base.grid_blocks, base.use_graph)
     rays_b = st["camera_rays"] + st["extension_rays"] + st["shadow_rays"]
     samples = sum(j[2] * j[3] * j[4] ** 2 for j in jobs)
