@@ -293,24 +293,28 @@ __global__ __launch_bounds__(BLOCK) void kExtend(PathState ps)
 __global__ __launch_bounds__(1024) void kPlanBins(PathState ps, uint32_t grid)
 {
     __shared__ uint32_t totals[kBins];
+    // The counter block, read once up front (no kernel writes it meanwhile), so
+    // its load overlaps the scan instead of a chain of read-modify-writes after it.
+    Counters c{};
+    if (threadIdx.x == 0) c = *ps.ctr;
     scanBlockCounts<kBins>(ps.block_count, ps.block_off, grid, true, totals);
     if (threadIdx.x == 0) {
         // The bookkeeping of the last finish and of this bounce's generation, both
         // already used by kGenerate and kExtend (pendingGeneration) and applied
         // here, where one thread runs after them anyway.
-        Counters* c = ps.ctr;
-        const Generation g = pendingGeneration(c);
-        c->paths_done += c->n_term;
-        c->shadow_rays += c->n_shadow;
-        c->n_term = 0;
-        c->n_shadow = 0;
-        c->n_free = g.free_top - g.n_new;
-        c->n_active += g.n_new;
-        c->work_next += g.n_new;
-        c->camera_rays += g.n_new;
-        c->extend_rays += c->n_active;
-        c->n_in = c->n_active;
-        for (int b = 0; b < kBins; ++b) c->bin_count[b] = totals[b];
+        const Generation g = pendingGeneration(&c);
+        c.paths_done += c.n_term;
+        c.shadow_rays += c.n_shadow;
+        c.n_term = 0;
+        c.n_shadow = 0;
+        c.n_free = g.free_top - g.n_new;
+        c.n_active += g.n_new;
+        c.work_next += g.n_new;
+        c.camera_rays += g.n_new;
+        c.extend_rays += c.n_active;
+        c.n_in = c.n_active;
+        for (int b = 0; b < kBins; ++b) c.bin_count[b] = totals[b];
+        *ps.ctr = c;
     }
 }
 
