@@ -179,18 +179,22 @@ __device__ __forceinline__ BoxRay makeBoxRay(float ox, float oy, float oz, float
     return b;
 }
 
+// f * (1 + 2^-20) for f >= 0 and f * (1 - 2^-20) for f < 0, i.e. f + |f| 2^-20
+// rounded once: one fused multiply-add gives the same value for every finite f
+// (and +inf for +inf; -inf, which no finite box yields, would give NaN and then
+// pass the box, never lose it).
 __device__ __forceinline__ float widenFar(float f)
 {
-    return f * (f < 0.0f ? (1.0f - 9.5367431640625e-07f) : (1.0f + 9.5367431640625e-07f));
+    return fmaf(fabsf(f), 9.5367431640625e-07f, f);
 }
 
-// One axis: tn takes the near distance, tf the (unwidened) far one.
+// One axis: the near and the (unwidened) far distance of the slab.
 __device__ __forceinline__ void slab(float lo, float hi, float o, float inv, float& tn, float& tf)
 {
 #pragma clang fp contract(off)
     const float t0 = (lo - o) * inv, t1 = (hi - o) * inv;
-    tn = fmaxf(tn, fminf(t0, t1));
-    tf = fminf(tf, fmaxf(t0, t1));
+    tn = fminf(t0, t1);
+    tf = fmaxf(t0, t1);
 }
 
 // widenFar and its rounding are non-decreasing, so widening the smallest far
@@ -198,15 +202,17 @@ __device__ __forceinline__ void slab(float lo, float hi, float o, float inv, flo
 __device__ __forceinline__ void testChildren(const DevNode& n, const BoxRay& r, float tlo, float thi, bool& h0,
                                              bool& h1, float& n0, float& n1)
 {
-    float tn0 = tlo, tf0 = INFINITY, tn1 = tlo, tf1 = INFINITY;
-    slab(n.a[0], n.a[1], r.ox, r.ix, tn0, tf0);
-    slab(n.a[2], n.a[3], r.oy, r.iy, tn0, tf0);
-    slab(n.b[0], n.b[1], r.oz, r.iz, tn0, tf0);
-    slab(n.b[2], n.b[3], r.ox, r.ix, tn1, tf1);
-    slab(n.c[0], n.c[1], r.oy, r.iy, tn1, tf1);
-    slab(n.c[2], n.c[3], r.oz, r.iz, tn1, tf1);
-    tf0 = fminf(thi, widenFar(tf0));
-    tf1 = fminf(thi, widenFar(tf1));
+    float nx0, fx0, ny0, fy0, nz0, fz0, nx1, fx1, ny1, fy1, nz1, fz1;
+    slab(n.a[0], n.a[1], r.ox, r.ix, nx0, fx0);
+    slab(n.a[2], n.a[3], r.oy, r.iy, ny0, fy0);
+    slab(n.b[0], n.b[1], r.oz, r.iz, nz0, fz0);
+    slab(n.b[2], n.b[3], r.ox, r.ix, nx1, fx1);
+    slab(n.c[0], n.c[1], r.oy, r.iy, ny1, fy1);
+    slab(n.c[2], n.c[3], r.oz, r.iz, nz1, fz1);
+    const float tn0 = fmaxf(fmaxf(tlo, nx0), fmaxf(ny0, nz0));
+    const float tn1 = fmaxf(fmaxf(tlo, nx1), fmaxf(ny1, nz1));
+    const float tf0 = fminf(thi, widenFar(fminf(fminf(fx0, fy0), fz0)));
+    const float tf1 = fminf(thi, widenFar(fminf(fminf(fx1, fy1), fz1)));
     h0 = tn0 <= tf0;
     h1 = tn1 <= tf1 && n.child[1] != kNoChild;
     n0 = tn0;
@@ -258,7 +264,7 @@ __device__ __forceinline__ bool traverseBlas(const DevScene& sc, int32_t root, u
             const DevNode n = loadNode(sc.nodes, ref);
             bool h0, h1;
             float t0, t1;
-            const float lim = ANY ? cullLimit(r.tmax) : cullLimit(best.object == kMissId ? r.tmax : best.t);
+            const float lim = cullLimit(ANY ? r.tmax : best.t);
             testChildren(n, br, r.tmin, lim, h0, h1, t0, t1);
             if (h0 && h1) {
                 const bool swap = t1 < t0;
@@ -361,7 +367,7 @@ __device__ __forceinline__ bool traceRay(const DevScene& sc, const RayF& r, HitF
             const DevNode n = loadNode(sc.nodes, ref);
             bool h0, h1;
             float t0, t1;
-            const float lim = ANY ? cullLimit(r.tmax) : cullLimit(best.object == kMissId ? r.tmax : best.t);
+            const float lim = cullLimit(ANY ? r.tmax : best.t);
             testChildren(n, br, r.tmin, lim, h0, h1, t0, t1);
             if (h0 && h1) {
                 const bool swap = t1 < t0;
@@ -407,6 +413,7 @@ struct RayQuery {
     BoxRay box;      // box-test form of the ray of the current level (world or instance space)
     ShearRay sh;     // triangle-test form of the instance-space ray
     HitF best;
+    float lim;       // box-culling limit: cullLimit of best.t, updated with it
     int32_t ref;
     int32_t pending; // the current TLAS leaf's objects not yet taken, or kNoChild
     int sp, sp_blas; // stack depth, and where the current instance's entries start
@@ -420,6 +427,7 @@ struct RayQuery {
         best.object = kMissId;
         best.prim = kMissId;
         best.b1 = best.b2 = 0.0f;
+        lim = cullLimit(r.tmax);
         box = makeBoxRay(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz);
         ref = sc.tlas_root;
         pending = kNoChild;
@@ -462,7 +470,6 @@ struct RayQuery {
         const DevNode n = loadNode(sc.nodes, ref);
         bool h0, h1;
         float t0, t1;
-        const float lim = ANY ? cullLimit(r.tmax) : cullLimit(best.object == kMissId ? r.tmax : best.t);
         testChildren(n, box, r.tmin, lim, h0, h1, t0, t1);
         if (h0 && h1) {
             const bool swap = t1 < t0;
@@ -499,6 +506,7 @@ struct RayQuery {
                         // Any-hit answers only whether something was hit.
                         best.b1 = ANY ? 0.0f : v / det;
                         best.b2 = ANY ? 0.0f : w / det;
+                        if (!ANY) lim = cullLimit(t);
                     }
                 }
             }
@@ -533,6 +541,7 @@ struct RayQuery {
                     best.prim = kMissId;
                     best.b1 = 0.0f;
                     best.b2 = 0.0f;
+                    if (!ANY) lim = cullLimit(tf);
                 }
             }
             if (found()) {
