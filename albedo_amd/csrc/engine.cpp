@@ -27,6 +27,9 @@ constexpr uint32_t kMaxDepthLimit = uint32_t((1024 - 2) / 7);
 // Samples of one pixel generated back to back (a pass of the work order).
 constexpr uint32_t kPixelGroup = 16;
 
+// Render accumulators up to this size stay allocated between renders.
+constexpr size_t kKeepAccumBytes = size_t(256) << 20;
+
 // Blocks of the grid-stride queue kernels: the configured count, or by default
 // enough for one pass over a full queue up to 2048 blocks (8 per CU).
 uint32_t gridFor(uint32_t configured, uint32_t capacity, uint32_t block)
@@ -131,10 +134,13 @@ void Engine::freeAll()
     graph_ = nullptr;
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (arena_) (void)hipFree(arena_);
+    if (accum_) (void)hipFree(accum_);
     if (h_counters_) (void)hipHostFree(h_counters_);
     if (h_segs_) (void)hipHostFree(h_segs_);
     if (stream_) (void)hipStreamDestroy(stream_);
     arena_ = nullptr;
+    accum_ = nullptr;
+    accum_cap_ = 0;
     h_counters_ = nullptr;
     h_segs_ = nullptr;
     d_segs_ = nullptr;
@@ -306,8 +312,36 @@ ab_status Engine::renderBatch(const std::vector<JobSpec>& specs, const std::vect
     const uint32_t capacity = std::max<uint32_t>(cfg.paths_in_flight, 1024);
     ab_status st = reserve(capacity, uint32_t(specs.size()), nullptr);
     if (st != AB_OK) return st;
+    // Accumulators up to kKeepAccumBytes stay allocated between renders (an
+    // allocation and its free cost about as much host time as a small render's
+    // GPU work); larger ones are allocated for this call only.
+    struct OneShot {
+        void* p{nullptr};
+        hipStream_t s{nullptr};
+        ~OneShot()
+        {
+            if (!p) return;
+            (void)hipStreamSynchronize(s);
+            (void)hipFree(p);
+        }
+    } one_shot;
     void* acc = nullptr;
-    HIP_TRY(hipMallocAsync(&acc, accum_bytes, stream_), "hipMalloc(accumulators)");
+    if (accum_bytes <= kKeepAccumBytes) {
+        if (accum_bytes > accum_cap_) {
+            // hipFree waits for the device's queued work, so the old buffer is idle when it goes.
+            if (accum_) (void)hipFree(accum_);
+            accum_ = nullptr;
+            accum_cap_ = 0;
+            const size_t want = std::min(accum_bytes + accum_bytes / 4, kKeepAccumBytes);
+            HIP_TRY(hipMalloc(&accum_, want), "hipMalloc(accumulators)");
+            accum_cap_ = want;
+        }
+        acc = accum_;
+    } else {
+        HIP_TRY(hipMalloc(&acc, accum_bytes), "hipMalloc(accumulators)");
+        one_shot.p = acc;
+        one_shot.s = stream_;
+    }
     HIP_TRY(hipMemsetAsync(acc, 0, accum_bytes, stream_), "hipMemset(accumulators)");
     size_t off = 0;
     for (size_t i = 0; i < jobs_.size(); ++i) {
@@ -356,8 +390,9 @@ ab_status Engine::renderBatch(const std::vector<JobSpec>& specs, const std::vect
     (void)hipEventElapsedTime(&ms, e0, e1);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    (void)hipFreeAsync(acc, stream_);
-    (void)hipStreamSynchronize(stream_);
+    // On success the image copies above synchronised the stream; after a failure
+    // the next render must not find work of this one still queued.
+    if (st != AB_OK) (void)hipStreamSynchronize(stream_);
     if (st == AB_OK && stats) {
         stats->camera_rays = c.camera_rays;
         stats->extension_rays = c.extend_rays - c.camera_rays;

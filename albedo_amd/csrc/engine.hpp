@@ -90,6 +90,8 @@ private:
     ab_context* ctx_;
     hipStream_t stream_{nullptr};
     void* arena_{nullptr};
+    void* accum_{nullptr};   // render accumulators kept between renders (up to kKeepAccumBytes)
+    size_t accum_cap_{0};
     uint32_t capacity_{0};
     uint32_t max_jobs_{0};
     PathState ps_{};
