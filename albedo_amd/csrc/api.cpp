@@ -197,7 +197,11 @@ ab_status ab_scene_create(ab_context* ctx, const ab_scene_desc* desc, ab_scene**
     s->dev.sky_light = hs.sky_light;
     s->dev.stack_depth = stack;
     s->dev.has_spheres = 0;
-    for (const DevObject& o : hs.objects) s->dev.has_spheres |= o.kind == kObjSphere ? 1u : 0u;
+    s->dev.has_meshes = 0;
+    for (const DevObject& o : hs.objects) {
+        s->dev.has_spheres |= o.kind == kObjSphere ? 1u : 0u;
+        s->dev.has_meshes |= o.kind == kObjSphere ? 0u : 1u;
+    }
     s->tlas_depth = hs.tlas_depth;
     s->blas_depth = hs.max_blas_depth;
     s->tlas_nodes = hs.tlas_nodes;

@@ -49,6 +49,30 @@ LaunchShape shapeFor(const ab_launch_config& cfg, uint32_t capacity, uint32_t st
     return s;
 }
 
+// Rays per lane of a full queue in the lane-refill tracing kernels (as in the
+// ray-query kernel: slices of 6 rays per lane).
+constexpr uint32_t kQueueRaysPerLane = 6;
+
+// Extension and shadow rays are traced with lane refill when every job of the
+// render shares one scene and that scene has mesh instances: incoherent rays
+// through triangle BLASes vary widely in length. Sphere-only scenes keep the
+// one-ray-per-lane kernels, which were faster on them (shallow trees).
+void chooseTracing(LaunchShape& s, const std::vector<DevJob>& jobs, uint32_t capacity)
+{
+    if (jobs.empty() || !jobs[0].scene.has_meshes) return;
+    for (const DevJob& j : jobs)
+        if (std::memcmp(&j.scene, &jobs[0].scene, sizeof(DevScene)) != 0) return;
+    s.refill = 1;
+    s.scene = jobs[0].scene;
+    s.qgrid = std::max<uint32_t>((capacity + 64 * kQueueRaysPerLane - 1) / (64 * kQueueRaysPerLane), 1);
+}
+
+bool sameTracing(const LaunchShape& a, const LaunchShape& b)
+{
+    if (a.refill != b.refill) return false;
+    return !a.refill || (a.qgrid == b.qgrid && std::memcmp(&a.scene, &b.scene, sizeof(DevScene)) == 0);
+}
+
 #define HIP_TRY(expr, what)                                   \
     do {                                                      \
         const hipError_t e_ = (expr);                         \
@@ -231,7 +255,7 @@ ab_status Engine::capture(const LaunchShape& shape)
     // The graph bakes in the path-state pointers, which change only when
     // reserve() reallocates (and then drops the graph).
     if (graph_ && shape.block == graph_shape_.block && shape.grid == graph_shape_.grid &&
-        shape.stack == graph_shape_.stack)
+        shape.stack == graph_shape_.stack && sameTracing(shape, graph_shape_))
         return AB_OK;
     if (graph_) {
         (void)hipGraphExecDestroy(graph_);
@@ -364,7 +388,8 @@ ab_status Engine::renderBatch(const std::vector<JobSpec>& specs, const std::vect
     seg_hi_ = seg_sent_ = uint32_t(jobs_.size());
     HIP_TRY(hipMemcpyAsync(d_segs_, h_segs_, sizeof(DevSegment) * jobs_.size(), hipMemcpyHostToDevice, stream_),
             "hipMemcpy(segments)");
-    const LaunchShape shape = shapeFor(cfg, capacity, stack);
+    LaunchShape shape = shapeFor(cfg, capacity, stack);
+    chooseTracing(shape, jobs_, capacity);
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0), "hipEventCreate");
     HIP_TRY(hipEventCreate(&e1), "hipEventCreate");

@@ -112,6 +112,7 @@ struct DevScene {
     int32_t sky_light;
     uint32_t stack_depth; // most traversal-stack entries a query holds (traceRay and RayQuery)
     uint32_t has_spheres; // any object is a sphere (else ray queries skip the FP64 test)
+    uint32_t has_meshes;  // any object is a mesh instance (the path tracer then traces with lane refill)
 };
 
 } // namespace ab

@@ -759,6 +759,86 @@ __global__ __launch_bounds__(BLOCK) void kShadow(PathState ps)
     }
 }
 
+// ------------------------------------------------- extend and shadow, lane refill
+
+// For renders whose jobs share one scene with mesh instances (passed as a kernel
+// argument): the extension and shadow rays traced as the ray-query kernel
+// traces them. Each one-wave block owns a contiguous slice of the queue, and a
+// lane whose ray has finished takes the next ray of the slice once kQueueRefill
+// lanes are free, so a wave no longer waits for its longest ray. RayQuery
+// visits the same nodes and primitives as traceRay, so the hits and shadow
+// results are those of kExtend and kShadow. The material keys are counted
+// afterwards by kCountBins, over the block partition kScatterBins uses.
+constexpr uint32_t kQueueRefill = 16; // free lanes that trigger a refill
+constexpr uint32_t kQueueSteps = 16;  // traversal steps between refill checks
+
+template <int STACK, bool ANY, bool SPHERES, bool LEAF_IN_REG>
+__global__ __launch_bounds__(64) void kTraceQueue(PathState ps, DevScene sc)
+{
+    __shared__ int32_t stack[STACK * 64];
+    const Counters* c = ps.ctr;
+    const uint32_t n = ANY ? c->n_shadow : c->n_active + pendingGeneration(c).n_new;
+    const uint32_t per_lane = (n + 64 * gridDim.x - 1) / (64 * gridDim.x);
+    const uint32_t slice = per_lane * 64;
+    uint32_t next = xcdRemap(blockIdx.x, gridDim.x) * slice;
+    const uint32_t end = next + slice < n ? next + slice : n;
+    const uint32_t lane = __lane_id();
+    const LdsStack<64> stk{stack + threadIdx.x};
+    RayQuery<64, ANY, SPHERES, LEAF_IN_REG> q;
+    constexpr uint32_t kNone = ~0u;
+    uint32_t id = kNone, slot = 0;
+    while (true) {
+        const bool idle = id == kNone;
+        const unsigned long long free_lanes = __ballot(idle);
+        if (next < end && (__popcll(free_lanes) >= kQueueRefill || free_lanes == ~0ull)) {
+            const uint32_t mine = next + uint32_t(__popcll(free_lanes & ((1ull << lane) - 1ull)));
+            next += uint32_t(__popcll(free_lanes));
+            if (idle && mine < end) {
+                slot = ANY ? ps.shadowq[mine] : ps.active[mine];
+                const float4 o = ANY ? ps.sh_org[slot] : ps.org[slot];
+                const float4 d = ANY ? ps.sh_dir[slot] : ps.dir[slot];
+                q.begin(sc, RayF{o.x, o.y, o.z, 0.0f, d.x, d.y, d.z, ANY ? o.w : __builtin_huge_valf()});
+                id = mine;
+            }
+        }
+        if (__ballot(id != kNone) == 0) break;
+        if (id != kNone) {
+            for (uint32_t k = 0; k < kQueueSteps && !q.done; ++k) q.step(sc, stk);
+            if (q.done) {
+                if (ANY) {
+                    if (q.best.object == kMissId) {
+                        const float4 cc = ps.sh_c[slot];
+                        float4 rr = ps.rad[slot];
+                        rr.x += cc.x;
+                        rr.y += cc.y;
+                        rr.z += cc.z;
+                        ps.rad[slot] = rr;
+                    }
+                } else {
+                    const HitF& h = q.best;
+                    ps.hit[slot] = make_uint4(__float_as_uint(h.t), h.object, h.prim, __float_as_uint(h.b1));
+                    ps.hit_b2[slot] = h.b2;
+                    ps.key[id] = 1u << materialBin(sc, h.object);
+                }
+                id = kNone;
+            }
+        }
+    }
+}
+
+// Per-block material counts of the keys kTraceQueue wrote (kExtend's counting).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void kCountBins(PathState ps)
+{
+    const uint32_t n = ps.ctr->n_active + pendingGeneration(ps.ctr).n_new;
+    WaveCounts<kBins> wc;
+    for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
+        const uint32_t i = base + threadIdx.x;
+        wc.add(i < n ? ps.key[i] : 0u);
+    }
+    storeBlockCounts<BLOCK, kBins>(wc, ps.block_count);
+}
+
 // ------------------------------------------------------------------ finish
 
 __device__ __forceinline__ unsigned long long toFixed(float v)
@@ -879,38 +959,62 @@ __global__ void kAddWork(PathState ps, unsigned long long total, uint32_t seg_lo
     c->seg_hi = seg_hi;
 }
 
-template <int BLOCK, int STACK>
-hipError_t bounce(const PathState& ps, uint32_t grid, hipStream_t s)
+// The lane-refill tracing kernel for the shape's scene: with the FP64 sphere
+// test only if the scene has spheres, and the rest of a TLAS leaf on the stack
+// when that extra entry fits the stack class, else in a register.
+template <int STACK, bool ANY>
+void launchTraceQueue(const PathState& ps, const LaunchShape& sh, hipStream_t s)
 {
+    const bool reg = sh.scene.stack_depth + 1 > uint32_t(STACK);
+    const dim3 g(sh.qgrid), b(64);
+    if (sh.scene.has_spheres) {
+        if (reg) hipLaunchKernelGGL((kTraceQueue<STACK, ANY, true, true>), g, b, 0, s, ps, sh.scene);
+        else hipLaunchKernelGGL((kTraceQueue<STACK, ANY, true, false>), g, b, 0, s, ps, sh.scene);
+    } else {
+        if (reg) hipLaunchKernelGGL((kTraceQueue<STACK, ANY, false, true>), g, b, 0, s, ps, sh.scene);
+        else hipLaunchKernelGGL((kTraceQueue<STACK, ANY, false, false>), g, b, 0, s, ps, sh.scene);
+    }
+}
+
+template <int BLOCK, int STACK>
+hipError_t bounce(const PathState& ps, const LaunchShape& sh, hipStream_t s)
+{
+    const uint32_t grid = sh.grid;
     hipLaunchKernelGGL(kGenerate, dim3(grid), dim3(BLOCK), 0, s, ps);
-    hipLaunchKernelGGL((kExtend<BLOCK, STACK>), dim3(grid), dim3(BLOCK), 0, s, ps);
+    if (sh.refill) {
+        launchTraceQueue<STACK, false>(ps, sh, s);
+        hipLaunchKernelGGL((kCountBins<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
+    } else {
+        hipLaunchKernelGGL((kExtend<BLOCK, STACK>), dim3(grid), dim3(BLOCK), 0, s, ps);
+    }
     hipLaunchKernelGGL(kPlanBins, dim3(1), dim3(1024), 0, s, ps, grid);
     hipLaunchKernelGGL((kScatterBins<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     hipLaunchKernelGGL((kShade<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     hipLaunchKernelGGL((kCountShade<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     hipLaunchKernelGGL((kScatterShade<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
-    hipLaunchKernelGGL((kShadow<BLOCK, STACK>), dim3(grid), dim3(BLOCK), 0, s, ps);
+    if (sh.refill) launchTraceQueue<STACK, true>(ps, sh, s);
+    else hipLaunchKernelGGL((kShadow<BLOCK, STACK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     hipLaunchKernelGGL((kFinish<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     return hipGetLastError();
 }
 
 template <int BLOCK, int STACK>
-hipError_t bouncePair(const PathState& ps, uint32_t grid, hipStream_t s)
+hipError_t bouncePair(const PathState& ps, const LaunchShape& sh, hipStream_t s)
 {
     PathState swapped = ps;
     swapped.active = ps.active_next;
     swapped.active_next = ps.active;
-    const hipError_t e = bounce<BLOCK, STACK>(ps, grid, s);
+    const hipError_t e = bounce<BLOCK, STACK>(ps, sh, s);
     if (e != hipSuccess) return e;
-    return bounce<BLOCK, STACK>(swapped, grid, s);
+    return bounce<BLOCK, STACK>(swapped, sh, s);
 }
 
 template <int BLOCK>
 hipError_t bounceB(const PathState& ps, const LaunchShape& sh, hipStream_t s)
 {
-    if (sh.stack <= kShortStack) return bouncePair<BLOCK, kShortStack>(ps, sh.grid, s);
-    if (sh.stack <= 32) return bouncePair<BLOCK, 32>(ps, sh.grid, s);
-    if (sh.stack <= 64) return bouncePair<BLOCK, 64>(ps, sh.grid, s);
+    if (sh.stack <= kShortStack) return bouncePair<BLOCK, kShortStack>(ps, sh, s);
+    if (sh.stack <= 32) return bouncePair<BLOCK, 32>(ps, sh, s);
+    if (sh.stack <= 64) return bouncePair<BLOCK, 64>(ps, sh, s);
     return hipErrorInvalidValue;
 }
 
@@ -922,8 +1026,8 @@ hipError_t launchBouncePair(const PathState& ps, const LaunchShape& sh, hipStrea
     case 64: return bounceB<64>(ps, sh, s);
     case 128: return bounceB<128>(ps, sh, s);
     case 256:
-        if (sh.stack <= kShortStack) return bouncePair<256, kShortStack>(ps, sh.grid, s);
-        if (sh.stack <= 32) return bouncePair<256, 32>(ps, sh.grid, s);
+        if (sh.stack <= kShortStack) return bouncePair<256, kShortStack>(ps, sh, s);
+        if (sh.stack <= 32) return bouncePair<256, 32>(ps, sh, s);
         return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
     }

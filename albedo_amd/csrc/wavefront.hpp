@@ -108,6 +108,9 @@ struct LaunchShape {
     uint32_t block;      // 64, 128 or 256
     uint32_t grid;       // blocks of the grid-stride kernels
     uint32_t stack;      // traversal stack depth (max over the jobs' scenes)
+    uint32_t refill{0};  // trace with the lane-refill kernels over `scene` (every job's scene)
+    uint32_t qgrid{0};   // their one-wave blocks
+    DevScene scene{};
 };
 
 // Enqueues two bounces (generate, extend, bin, shade, shadow, finish, with the
