@@ -53,24 +53,38 @@ LaunchShape shapeFor(const ab_launch_config& cfg, uint32_t capacity, uint32_t st
 // ray-query kernel: slices of 6 rays per lane).
 constexpr uint32_t kQueueRaysPerLane = 6;
 
-// Extension and shadow rays are traced with lane refill when every job of the
-// render shares one scene and that scene has mesh instances: incoherent rays
-// through triangle BLASes vary widely in length. Sphere-only scenes keep the
-// one-ray-per-lane kernels, which were faster on them (shallow trees).
-void chooseTracing(LaunchShape& s, const std::vector<DevJob>& jobs, uint32_t capacity)
+// Extension and shadow rays are traced with lane refill where incoherent rays
+// through triangle BLASes dominate: over the one scene of every job when that
+// scene has mesh instances (`one_scene`: never for the serving table, whose
+// scenes come and go); in a queue of several scenes, the rays of mesh-only
+// scenes (split), each lane over its job's scene. Rays of scenes with spheres
+// keep the one-ray-per-lane kernels, which were faster on them (shallow trees
+// of spheres: refill cost the sphere room 50 %, and the serving mix 11 % when
+// it took every ray).
+void chooseTracing(LaunchShape& s, const SceneSet& set, uint32_t capacity, bool one_scene)
 {
-    if (jobs.empty() || !jobs[0].scene.has_meshes) return;
-    for (const DevJob& j : jobs)
-        if (std::memcmp(&j.scene, &jobs[0].scene, sizeof(DevScene)) != 0) return;
-    s.refill = 1;
-    s.scene = jobs[0].scene;
-    s.qgrid = std::max<uint32_t>((capacity + 64 * kQueueRaysPerLane - 1) / (64 * kQueueRaysPerLane), 1);
+    const uint32_t qgrid =
+        std::max<uint32_t>((capacity + 64 * kQueueRaysPerLane - 1) / (64 * kQueueRaysPerLane), 1);
+    if (one_scene && set.same && set.meshes) {
+        s.refill = kTraceOne;
+        s.scene = set.first;
+        s.q_spheres = set.first.has_spheres ? 1u : 0u;
+        s.q_reg = set.first.stack_depth + 1 > s.stack ? 1u : 0u;
+        s.qgrid = qgrid;
+    } else if (set.mesh_only) {
+        s.refill = kTraceSplit;
+        s.q_spheres = 0;
+        s.q_reg = set.deepest_mesh_only + 1 > s.stack ? 1u : 0u;
+        s.qgrid = qgrid;
+    }
 }
 
 bool sameTracing(const LaunchShape& a, const LaunchShape& b)
 {
     if (a.refill != b.refill) return false;
-    return !a.refill || (a.qgrid == b.qgrid && std::memcmp(&a.scene, &b.scene, sizeof(DevScene)) == 0);
+    if (a.refill == kTracePlain) return true;
+    return a.qgrid == b.qgrid && a.q_spheres == b.q_spheres && a.q_reg == b.q_reg &&
+           std::memcmp(&a.scene, &b.scene, sizeof(DevScene)) == 0;
 }
 
 #define HIP_TRY(expr, what)                                   \
@@ -389,7 +403,9 @@ ab_status Engine::renderBatch(const std::vector<JobSpec>& specs, const std::vect
     HIP_TRY(hipMemcpyAsync(d_segs_, h_segs_, sizeof(DevSegment) * jobs_.size(), hipMemcpyHostToDevice, stream_),
             "hipMemcpy(segments)");
     LaunchShape shape = shapeFor(cfg, capacity, stack);
-    chooseTracing(shape, jobs_, capacity);
+    SceneSet set;
+    for (const DevJob& j : jobs_) set.add(j.scene);
+    chooseTracing(shape, set, capacity, true);
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0), "hipEventCreate");
     HIP_TRY(hipEventCreate(&e1), "hipEventCreate");
@@ -446,6 +462,7 @@ ab_status Engine::beginStream(const ab_launch_config& cfg, uint32_t max_jobs)
 ab_status Engine::setStreamStack(uint32_t stack)
 {
     shape_ = shapeFor(stream_cfg_, capacity_, stack);
+    chooseTracing(shape_, table_set_, capacity_, false);
     return capture(shape_);
 }
 
@@ -455,11 +472,20 @@ ab_status Engine::appendJob(const DevJob& prepared, unsigned long long* accum, u
     if (tableFull()) return fail(AB_REJECTED_BUDGET, "job table full");
     if (prepared.scene.stack_depth > 64)
         return fail(AB_INVALID_ARGUMENT, "scene BVH too deep for the wavefront engine (stack > 64)");
-    // The bounce kernels follow the deepest scene in the table; queues do not
-    // depend on them, so they can change between bounces.
-    if (stackClass(prepared.scene.stack_depth) > shape_.stack) {
-        const ab_status st = setStreamStack(prepared.scene.stack_depth);
-        if (st != AB_OK) return st;
+    // The bounce kernels follow the scenes in the table (the deepest one's stack
+    // class; lane refill once one has meshes); queues do not depend on them, so
+    // they can change between bounces.
+    const SceneSet before = table_set_;
+    table_set_.add(prepared.scene);
+    const uint32_t stack = std::max(shape_.stack, stackClass(prepared.scene.stack_depth));
+    LaunchShape next = shapeFor(stream_cfg_, capacity_, stack);
+    chooseTracing(next, table_set_, capacity_, false);
+    if (next.stack != shape_.stack || !sameTracing(next, shape_)) {
+        const ab_status st = setStreamStack(stack);
+        if (st != AB_OK) {
+            table_set_ = before;
+            return st;
+        }
     }
     DevJob dj = prepared;
     dj.accum = accum;
@@ -530,7 +556,8 @@ ab_status Engine::resetTable()
     work_total_ = 0;
     seg_lo_ = seg_hi_ = seg_sent_ = 0;
     HIP_TRY(launchReset(ps_, 0, 0, stream_), "reset launch");
-    return shape_.stack > kShortStack ? setStreamStack(kShortStack) : AB_OK;
+    table_set_ = SceneSet{};
+    return shape_.stack > kShortStack || shape_.refill != kTracePlain ? setStreamStack(kShortStack) : AB_OK;
 }
 
 Engine& contextEngine(ab_context* ctx)

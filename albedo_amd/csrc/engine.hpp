@@ -5,6 +5,7 @@
 #include "context.hpp"
 #include "wavefront.hpp"
 
+#include <cstring>
 #include <vector>
 
 namespace ab {
@@ -38,6 +39,24 @@ inline JobBuffers jobBuffers(const DevJob& j)
     b.total = b.mean_offset + align(npx * 12);
     return b;
 }
+
+// What the bounce kernels must handle of a set of jobs' scenes.
+struct SceneSet {
+    bool meshes{false}, mesh_only{false}, same{true}, empty{true};
+    uint32_t deepest_mesh_only{0}; // largest stack bound of the mesh-only scenes
+    DevScene first{};
+    void add(const DevScene& s)
+    {
+        if (empty) first = s;
+        else if (std::memcmp(&s, &first, sizeof(DevScene)) != 0) same = false;
+        empty = false;
+        meshes = meshes || s.has_meshes;
+        if (s.has_meshes && !s.has_spheres) {
+            mesh_only = true;
+            deepest_mesh_only = deepest_mesh_only > s.stack_depth ? deepest_mesh_only : s.stack_depth;
+        }
+    }
+};
 
 class Engine {
 public:
@@ -105,6 +124,7 @@ private:
     std::vector<DevJob> jobs_;
     unsigned long long work_total_{0};
     LaunchShape shape_{};          // serving: current bounce shape
+    SceneSet table_set_{};         // serving: the scenes of the job table
     ab_launch_config stream_cfg_{}; // serving: configuration the stream started with
     hipGraphExec_t graph_{nullptr};
     LaunchShape graph_shape_{};     // shape graph_ was captured with

@@ -262,7 +262,12 @@ __device__ __forceinline__ uint32_t materialBin(const DevScene& sc, uint32_t obj
     return sc.materials[sc.objects[obj].material].kind + 1;
 }
 
-template <int BLOCK, int STACK>
+// Scenes whose rays kTraceQueue takes in a queue of several scenes (SPLIT).
+__device__ __forceinline__ bool meshOnly(const DevScene& sc) { return sc.has_meshes && !sc.has_spheres; }
+
+// SPLIT: rays of mesh-only scenes are left to kTraceQueue (their keys too; the
+// block counts then come from kCountBins).
+template <int BLOCK, int STACK, bool SPLIT>
 __global__ __launch_bounds__(BLOCK) void kExtend(PathState ps)
 {
     __shared__ int32_t stack[STACK * BLOCK];
@@ -272,10 +277,10 @@ __global__ __launch_bounds__(BLOCK) void kExtend(PathState ps)
     for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
         const uint32_t i = base + threadIdx.x;
         uint32_t flags = 0;
-        if (i < n) {
-            const uint32_t slot = ps.active[i];
+        const uint32_t slot = i < n ? ps.active[i] : 0u;
+        const uint32_t j = i < n ? __float_as_uint(ps.prev[slot].w) : 0u;
+        if (i < n && !(SPLIT && meshOnly(ps.jobs[j].scene))) {
             const float4 o = ps.org[slot], d = ps.dir[slot];
-            const uint32_t j = __float_as_uint(ps.prev[slot].w);
             const DevScene& sc = ps.jobs[j].scene;
             const RayF r{o.x, o.y, o.z, 0.0f, d.x, d.y, d.z, __builtin_huge_valf()};
             HitF h;
@@ -737,15 +742,17 @@ __global__ __launch_bounds__(BLOCK) void kScatterShade(PathState ps)
 
 // Occlusion of the deferred light samples (reference Scene::occluded,
 // scene/scene.cpp:157-165); unoccluded contributions join the path's radiance.
-template <int BLOCK, int STACK>
+// SPLIT: rays of mesh-only scenes are left to kTraceQueue.
+template <int BLOCK, int STACK, bool SPLIT>
 __global__ __launch_bounds__(BLOCK) void kShadow(PathState ps)
 {
     __shared__ int32_t stack[STACK * BLOCK];
     const uint32_t n = ps.ctr->n_shadow;
     for (uint32_t i = gridStart(); i < n; i += gridStride()) {
         const uint32_t slot = ps.shadowq[i];
-        const float4 o = ps.sh_org[slot], d = ps.sh_dir[slot];
         const uint32_t j = __float_as_uint(ps.prev[slot].w);
+        if (SPLIT && meshOnly(ps.jobs[j].scene)) continue;
+        const float4 o = ps.sh_org[slot], d = ps.sh_dir[slot];
         const RayF r{o.x, o.y, o.z, 0.0f, d.x, d.y, d.z, o.w};
         HitF h;
         if (!traceRay<BLOCK, true>(ps.jobs[j].scene, r, h, LdsStack<BLOCK>{stack + threadIdx.x})) {
@@ -761,9 +768,10 @@ __global__ __launch_bounds__(BLOCK) void kShadow(PathState ps)
 
 // ------------------------------------------------- extend and shadow, lane refill
 
-// For renders whose jobs share one scene with mesh instances (passed as a kernel
-// argument): the extension and shadow rays traced as the ray-query kernel
-// traces them. Each one-wave block owns a contiguous slice of the queue, and a
+// For queues with mesh-instance scenes: the extension and shadow rays traced as
+// the ray-query kernel traces them, over one scene passed as a kernel argument
+// or (MIXED, a queue of several scenes) the rays of mesh-only scenes, each lane
+// over its job's scene; kExtend and kShadow take the others. Each one-wave block owns a contiguous slice of the queue, and a
 // lane whose ray has finished takes the next ray of the slice once kQueueRefill
 // lanes are free, so a wave no longer waits for its longest ray. RayQuery
 // visits the same nodes and primitives as traceRay, so the hits and shadow
@@ -772,8 +780,15 @@ __global__ __launch_bounds__(BLOCK) void kShadow(PathState ps)
 constexpr uint32_t kQueueRefill = 16; // free lanes that trigger a refill
 constexpr uint32_t kQueueSteps = 16;  // traversal steps between refill checks
 
-template <int STACK, bool ANY, bool SPHERES, bool LEAF_IN_REG>
-__global__ __launch_bounds__(64) void kTraceQueue(PathState ps, DevScene sc)
+template <bool MIXED>
+__device__ __forceinline__ const DevScene& laneScene(const PathState& ps, uint32_t job, const DevScene& one)
+{
+    if constexpr (MIXED) return ps.jobs[job].scene;
+    else return one;
+}
+
+template <int STACK, bool ANY, bool SPHERES, bool LEAF_IN_REG, bool MIXED>
+__global__ __launch_bounds__(64) void kTraceQueue(PathState ps, DevScene one)
 {
     __shared__ int32_t stack[STACK * 64];
     const Counters* c = ps.ctr;
@@ -786,7 +801,7 @@ __global__ __launch_bounds__(64) void kTraceQueue(PathState ps, DevScene sc)
     const LdsStack<64> stk{stack + threadIdx.x};
     RayQuery<64, ANY, SPHERES, LEAF_IN_REG> q;
     constexpr uint32_t kNone = ~0u;
-    uint32_t id = kNone, slot = 0;
+    uint32_t id = kNone, slot = 0, job = 0;
     while (true) {
         const bool idle = id == kNone;
         const unsigned long long free_lanes = __ballot(idle);
@@ -795,14 +810,19 @@ __global__ __launch_bounds__(64) void kTraceQueue(PathState ps, DevScene sc)
             next += uint32_t(__popcll(free_lanes));
             if (idle && mine < end) {
                 slot = ANY ? ps.shadowq[mine] : ps.active[mine];
-                const float4 o = ANY ? ps.sh_org[slot] : ps.org[slot];
-                const float4 d = ANY ? ps.sh_dir[slot] : ps.dir[slot];
-                q.begin(sc, RayF{o.x, o.y, o.z, 0.0f, d.x, d.y, d.z, ANY ? o.w : __builtin_huge_valf()});
-                id = mine;
+                if (MIXED) job = __float_as_uint(ps.prev[slot].w);
+                if (!MIXED || meshOnly(ps.jobs[job].scene)) {
+                    const float4 o = ANY ? ps.sh_org[slot] : ps.org[slot];
+                    const float4 d = ANY ? ps.sh_dir[slot] : ps.dir[slot];
+                    q.begin(laneScene<MIXED>(ps, job, one),
+                            RayF{o.x, o.y, o.z, 0.0f, d.x, d.y, d.z, ANY ? o.w : __builtin_huge_valf()});
+                    id = mine;
+                }
             }
         }
         if (__ballot(id != kNone) == 0) break;
         if (id != kNone) {
+            const DevScene& sc = laneScene<MIXED>(ps, job, one);
             for (uint32_t k = 0; k < kQueueSteps && !q.done; ++k) q.step(sc, stk);
             if (q.done) {
                 if (ANY) {
@@ -959,21 +979,28 @@ __global__ void kAddWork(PathState ps, unsigned long long total, uint32_t seg_lo
     c->seg_hi = seg_hi;
 }
 
-// The lane-refill tracing kernel for the shape's scene: with the FP64 sphere
-// test only if the scene has spheres, and the rest of a TLAS leaf on the stack
-// when that extra entry fits the stack class, else in a register.
+// The lane-refill tracing kernel for the shape: one scene or each lane's job's
+// scene (MIXED), with the FP64 sphere test only if a scene has spheres, and the
+// rest of a TLAS leaf in a register when that extra stack entry would not fit
+// the stack class.
+template <int STACK, bool ANY, bool MIXED>
+void launchTraceQueueM(const PathState& ps, const LaunchShape& sh, hipStream_t s)
+{
+    const dim3 g(sh.qgrid), b(64);
+    if (sh.q_spheres) {
+        if (sh.q_reg) hipLaunchKernelGGL((kTraceQueue<STACK, ANY, true, true, MIXED>), g, b, 0, s, ps, sh.scene);
+        else hipLaunchKernelGGL((kTraceQueue<STACK, ANY, true, false, MIXED>), g, b, 0, s, ps, sh.scene);
+    } else {
+        if (sh.q_reg) hipLaunchKernelGGL((kTraceQueue<STACK, ANY, false, true, MIXED>), g, b, 0, s, ps, sh.scene);
+        else hipLaunchKernelGGL((kTraceQueue<STACK, ANY, false, false, MIXED>), g, b, 0, s, ps, sh.scene);
+    }
+}
+
 template <int STACK, bool ANY>
 void launchTraceQueue(const PathState& ps, const LaunchShape& sh, hipStream_t s)
 {
-    const bool reg = sh.scene.stack_depth + 1 > uint32_t(STACK);
-    const dim3 g(sh.qgrid), b(64);
-    if (sh.scene.has_spheres) {
-        if (reg) hipLaunchKernelGGL((kTraceQueue<STACK, ANY, true, true>), g, b, 0, s, ps, sh.scene);
-        else hipLaunchKernelGGL((kTraceQueue<STACK, ANY, true, false>), g, b, 0, s, ps, sh.scene);
-    } else {
-        if (reg) hipLaunchKernelGGL((kTraceQueue<STACK, ANY, false, true>), g, b, 0, s, ps, sh.scene);
-        else hipLaunchKernelGGL((kTraceQueue<STACK, ANY, false, false>), g, b, 0, s, ps, sh.scene);
-    }
+    if (sh.refill == kTraceSplit) launchTraceQueueM<STACK, ANY, true>(ps, sh, s);
+    else launchTraceQueueM<STACK, ANY, false>(ps, sh, s);
 }
 
 template <int BLOCK, int STACK>
@@ -981,19 +1008,24 @@ hipError_t bounce(const PathState& ps, const LaunchShape& sh, hipStream_t s)
 {
     const uint32_t grid = sh.grid;
     hipLaunchKernelGGL(kGenerate, dim3(grid), dim3(BLOCK), 0, s, ps);
-    if (sh.refill) {
+    if (sh.refill == kTracePlain) {
+        hipLaunchKernelGGL((kExtend<BLOCK, STACK, false>), dim3(grid), dim3(BLOCK), 0, s, ps);
+    } else {
+        if (sh.refill == kTraceSplit)
+            hipLaunchKernelGGL((kExtend<BLOCK, STACK, true>), dim3(grid), dim3(BLOCK), 0, s, ps);
         launchTraceQueue<STACK, false>(ps, sh, s);
         hipLaunchKernelGGL((kCountBins<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
-    } else {
-        hipLaunchKernelGGL((kExtend<BLOCK, STACK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     }
     hipLaunchKernelGGL(kPlanBins, dim3(1), dim3(1024), 0, s, ps, grid);
     hipLaunchKernelGGL((kScatterBins<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     hipLaunchKernelGGL((kShade<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     hipLaunchKernelGGL((kCountShade<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     hipLaunchKernelGGL((kScatterShade<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
-    if (sh.refill) launchTraceQueue<STACK, true>(ps, sh, s);
-    else hipLaunchKernelGGL((kShadow<BLOCK, STACK>), dim3(grid), dim3(BLOCK), 0, s, ps);
+    if (sh.refill == kTracePlain)
+        hipLaunchKernelGGL((kShadow<BLOCK, STACK, false>), dim3(grid), dim3(BLOCK), 0, s, ps);
+    else if (sh.refill == kTraceSplit)
+        hipLaunchKernelGGL((kShadow<BLOCK, STACK, true>), dim3(grid), dim3(BLOCK), 0, s, ps);
+    if (sh.refill != kTracePlain) launchTraceQueue<STACK, true>(ps, sh, s);
     hipLaunchKernelGGL((kFinish<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     return hipGetLastError();
 }

@@ -104,13 +104,21 @@ struct PathState {
     uint32_t seg_mask;
 };
 
+// How the extension and shadow rays are traced: one ray per lane (kernels
+// kExtend, kShadow); with lane refill (kTraceQueue) over the one scene of every
+// job; or split, the rays of mesh-only scenes with lane refill (each lane over
+// its job's scene) and the others one per lane.
+enum TraceMode : uint32_t { kTracePlain = 0, kTraceOne = 1, kTraceSplit = 2 };
+
 struct LaunchShape {
-    uint32_t block;      // 64, 128 or 256
-    uint32_t grid;       // blocks of the grid-stride kernels
-    uint32_t stack;      // traversal stack depth (max over the jobs' scenes)
-    uint32_t refill{0};  // trace with the lane-refill kernels over `scene` (every job's scene)
-    uint32_t qgrid{0};   // their one-wave blocks
-    DevScene scene{};
+    uint32_t block;        // 64, 128 or 256
+    uint32_t grid;         // blocks of the grid-stride kernels
+    uint32_t stack;        // traversal stack depth (max over the jobs' scenes)
+    uint32_t refill{kTracePlain};
+    uint32_t qgrid{0};     // one-wave blocks of the lane-refill kernels
+    uint32_t q_spheres{0}; // a scene kTraceQueue traces has spheres (the FP64 sphere test is compiled in)
+    uint32_t q_reg{0};     // such a scene's stack bound + 1 exceeds the class: TLAS leaf rest in a register
+    DevScene scene{};      // kTraceOne: the scene
 };
 
 // Enqueues two bounces (generate, extend, bin, shade, shadow, finish, with the
