@@ -132,6 +132,30 @@ def test_batched_jobs_equal_single_renders(ctx, room):
         assert np.array_equal(single.view(np.uint32), out.view(np.uint32))
 
 
+def test_split_tracing_equals_single_scene_tracing(ctx, room):
+    """A batch of a mesh-only scene (the trefoil) and sphere scenes traces the
+    trefoil's rays with lane refill, each lane reading its job's scene, and the
+    others one ray per lane; alone, the trefoil job is traced with lane refill
+    over its one scene and the room job one ray per lane. Images are identical."""
+    s, g, o = room
+    ts = W.trefoil_scene()[0]
+    ts.add_light(PointLight((2.0, 4.0, 9.0), (60.0, 60.0, 60.0)))
+    tref = ctx.scene(ts)
+    full = ctx.scene(W.full_scene())
+    cam = W.ROOM_CAMERA
+    tcam = CameraSettings((0.0, -0.5, 12.0), (0.0, -0.5, 0.0), (0, 1, 0), 40.0, 48 / 32)
+    jobs = [(tref, tcam, W.render_params(48, 32, 2, 6, 2, 3)),
+            (g, CameraSettings(cam.look_from, cam.look_at, (0, 1, 0), 38.0, 40 / 24), W.render_params(40, 24, 2, 8, 3, 4)),
+            (full, CameraSettings(cam.look_from, cam.look_at, (0, 1, 0), 38.0, 32 / 18), W.render_params(32, 18, 2, 8, 3, 5)),
+            (tref, tcam, W.render_params(48, 32, 3, 4, 1, 6))]
+    outs, st = batch(ctx, jobs)
+    assert st.camera_rays == sum(p.width * p.height * p.samples_per_axis ** 2 for _, _, p in jobs)
+    assert any(float(o_.max()) > 0.0 for o_ in (outs[0], outs[3]))  # the trefoil is lit and in view
+    for (gs, c, p), out in zip(jobs, outs):
+        single = gs.render(c, p.width, p.height, p.samples_per_axis, p.max_depth, p.roulette_start, p.seed)
+        assert np.array_equal(single.view(np.uint32), out.view(np.uint32))
+
+
 def test_full_scene_same_streams(ctx):
     s = W.full_scene()
     g = ctx.scene(s)
