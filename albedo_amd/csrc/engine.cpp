@@ -55,8 +55,8 @@ constexpr uint32_t kQueueRaysPerLane = 6;
 
 // Extension and shadow rays are traced with lane refill where incoherent rays
 // through triangle BLASes dominate: over the one scene of every job when that
-// scene has mesh instances (`one_scene`: never for the serving table, whose
-// scenes come and go); in a queue of several scenes, the rays of mesh-only
+// scene has mesh instances (`one_scene`: the set holds every scene whose paths
+// may be in flight); in a queue of several scenes, the rays of mesh-only
 // scenes (split), each lane over its job's scene. Rays of scenes with spheres
 // keep the one-ray-per-lane kernels, which were faster on them (shallow trees
 // of spheres: refill cost the sphere room 50 %, and the serving mix 11 % when
@@ -457,6 +457,15 @@ ab_status Engine::beginStream(const ab_launch_config& cfg, uint32_t max_jobs)
     use_graph_ = cfg.use_graph != 0;
     HIP_TRY(launchReset(ps_, 0, 0, stream_), "reset launch");
     return setStreamStack(kShortStack);
+}
+
+ab_status Engine::traceScenes(const SceneSet* live)
+{
+    LaunchShape next = shapeFor(stream_cfg_, capacity_, shape_.stack);
+    chooseTracing(next, live ? *live : table_set_, capacity_, live != nullptr);
+    if (sameTracing(next, shape_)) return AB_OK;
+    shape_ = next;
+    return capture(shape_);
 }
 
 ab_status Engine::setStreamStack(uint32_t stack)

@@ -89,6 +89,11 @@ public:
     unsigned long long workTotal() const { return work_total_; }
     ab_status cancelJob(uint32_t index);
     ab_status runBounces(uint32_t count); // rounded up to an even count
+    // Serving: the scenes whose paths may be in flight, when known exactly (every
+    // job with paths left among them), else null for every scene of the table.
+    // Their being one scene with meshes lets the bounces trace over that scene
+    // alone (as a render of one job does); recaptures the bounces if that changes.
+    ab_status traceScenes(const SceneSet* live);
     ab_status resetTable();
     bool tableFull() const { return jobs_.size() >= max_jobs_; }
     size_t tableSize() const { return jobs_.size(); }

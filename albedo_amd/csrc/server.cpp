@@ -97,6 +97,8 @@ struct Server {
     size_t rr{0};                       // SHARE: rotates which running job is served first
     uint32_t policy{AB_SCHED_SMALL_FIRST}; // guarded by mu
     unsigned long long* h_done{nullptr}; // pinned: finished-sample lanes of the running jobs
+    SceneSet live_set{};                // scenes of the running jobs (loop thread)
+    bool live_exact{false};             // no cancelled job's paths can be in flight
     size_t h_done_jobs{0};              // jobs h_done has room for
 
     Server(ab_context* c, uint64_t b) : ctx(c), budget(b), engine(c) {}
@@ -307,6 +309,17 @@ void Server::loop()
                 thread_status = sst;
                 break;
             }
+            // Paths in flight belong to the running jobs, and to cancelled ones
+            // until the engine drains (their buffers wait in `retired`); a job
+            // retires only once all its samples have finished.
+            live_exact = retired.empty();
+            live_set = SceneSet{};
+            for (uint64_t id : running) live_set.add(jobs[id]->prepared.scene);
+        }
+        if (engine.traceScenes(live_exact ? &live_set : nullptr) != AB_OK) {
+            std::lock_guard<std::mutex> lock(mu);
+            thread_status = AB_DEVICE_ERROR;
+            break;
         }
         const double t0 = msSince(ctx);
         Counters c{};

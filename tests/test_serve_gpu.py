@@ -124,6 +124,35 @@ def test_mix_with_million_triangle_scene_equals_alone(ctx, scenes, icosphere):
         assert np.array_equal(alone.view(np.uint32), img.view(np.uint32))
 
 
+def test_one_mesh_scene_in_flight_then_a_cancel_and_other_scenes(ctx, scenes, icosphere):
+    # While only jobs of the icosphere scene are in flight the bounces trace over
+    # that one scene; a cancelled job keeps every scene of the table traced until
+    # its paths drain, and a room job brings the split tracing back. Every
+    # finished job equals its render alone.
+    room, _ = scenes
+    _, ico, _ = icosphere
+    c = W.ICOSPHERE_CAMERA
+    ico_cam = CameraSettings(c.look_from, c.look_at, (0, 1, 0), c.vertical_fov_degrees, 96 / 54)
+    first = [(ico, ico_cam, 96, 54, 3, 8, 3, 31), (ico, ico_cam, 64, 36, 2, 8, 3, 32)]
+    later = [(room, cam(1.0), 24, 24, 2, 8, 3, 34), (ico, ico_cam, 48, 27, 2, 8, 3, 35)]
+    with Server(ctx, 1 << 31) as srv:
+        results = [srv.wait(i) for i in [srv.submit(*j) for j in first]]
+        big = srv.submit(ico, ico_cam, 640, 360, 16, 8, 3, 33)
+        for _ in range(200):
+            if srv.poll(big)[1]["samples_done"] > 0:
+                break
+            time.sleep(0.005)
+        srv.cancel(big)
+        ids = [srv.submit(*j) for j in later]
+        code, st, _ = srv.wait(big)
+        assert code == _abi.AB_CANCELLED and st["state"] == Server.CANCELLED
+        results += [srv.wait(i) for i in ids]
+    for (g, cm, w, h, n, d, rr, seed), (code, status, img) in zip(first + later, results):
+        assert code == _abi.AB_OK and status["samples_done"] == w * h * n * n
+        alone = g.render(cm, w, h, n, d, rr, seed)
+        assert np.array_equal(alone.view(np.uint32), img.view(np.uint32))
+
+
 @pytest.mark.parametrize("policy", [_abi.SCHED_FIFO, _abi.SCHED_SHARE, _abi.SCHED_SMALL_FIRST])
 def test_policies_keep_images_and_order_jobs(ctx, scenes, policy):
     room, full = scenes
