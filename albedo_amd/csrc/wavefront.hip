@@ -820,7 +820,11 @@ __global__ __launch_bounds__(64) void kTraceQueue(PathState ps, DevScene one)
                 }
             }
         }
-        if (__ballot(id != kNone) == 0) break;
+        // Done when no lane holds a ray and the slice is used up. (MIXED: a refill
+        // can leave every lane idle, all its rays being the other kernels', while
+        // the slice still holds rays; `next` is the same in all lanes and grows by
+        // 64 at each such pass, so the loop ends.)
+        if (__ballot(id != kNone) == 0 && next >= end) break;
         if (id != kNone) {
             const DevScene& sc = laneScene<MIXED>(ps, job, one);
             for (uint32_t k = 0; k < kQueueSteps && !q.done; ++k) q.step(sc, stk);

@@ -156,6 +156,37 @@ def test_split_tracing_equals_single_scene_tracing(ctx, room):
         assert np.array_equal(single.view(np.uint32), out.view(np.uint32))
 
 
+def test_split_tracing_with_full_queues_traces_every_ray(ctx, room):
+    """Split tracing with the queue full: 16 384 paths in flight, so each wave of
+    the lane-refill kernel owns a slice of 6 x 64 queue entries, and runs of
+    another scene's rays (which it skips) fill whole refills inside a slice. A
+    wave has to go on past such a refill to the rest of its slice: every job of
+    the batch equals its render alone."""
+    s, g, o = room
+    ts = W.trefoil_scene()[0]
+    ts.add_light(PointLight((2.0, 4.0, 9.0), (60.0, 60.0, 60.0)))
+    tref = ctx.scene(ts)
+    full = ctx.scene(W.full_scene())
+    cam = W.ROOM_CAMERA
+    tcam = CameraSettings((0.0, -0.5, 12.0), (0.0, -0.5, 0.0), (0, 1, 0), 40.0, 64 / 48)
+    rcam = CameraSettings(cam.look_from, cam.look_at, (0, 1, 0), 38.0, 64 / 48)
+    jobs = [(g, rcam, W.render_params(64, 48, 4, 8, 3, 41)), (tref, tcam, W.render_params(64, 48, 3, 6, 2, 42)),
+            (full, rcam, W.render_params(64, 48, 2, 8, 3, 43)), (g, rcam, W.render_params(48, 32, 3, 8, 3, 44)),
+            (tref, tcam, W.render_params(48, 32, 4, 6, 2, 45))]
+    base = ctx.launch_config()
+    try:
+        ctx.set_launch_config(128, 16384, 0, 1)
+        outs, st = batch(ctx, jobs)
+        singles = [gs.render(c, p.width, p.height, p.samples_per_axis, p.max_depth, p.roulette_start, p.seed)
+                   for gs, c, p in jobs]
+    finally:
+        ctx.set_launch_config(base.block_size, base.paths_in_flight, base.grid_blocks, base.use_graph)
+    assert st.camera_rays == sum(p.width * p.height * p.samples_per_axis ** 2 for _, _, p in jobs) > 4 * 16384
+    assert float(outs[1].max()) > 0.0  # the trefoil is lit and in view
+    for single, out in zip(singles, outs):
+        assert np.array_equal(single.view(np.uint32), out.view(np.uint32))
+
+
 def test_full_scene_same_streams(ctx):
     s = W.full_scene()
     g = ctx.scene(s)
