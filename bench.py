@@ -67,7 +67,7 @@ def path_tracing_leg(ctx, cpu=True):
     cam = CameraSettings(c.look_from, c.look_at, (0, 1, 0), 38.0, 320 / 180)
     g.render(cam, 320, 180, 4, 8, 3, 0)  # warm-up (graph capture, allocation)
     t0 = time.perf_counter()
-    reps = 5
+    reps = 20  # about 35 ms: enough renders that host jitter averages out
     st = None
     for r in range(reps):
         _, st = g.render(cam, 320, 180, 4, 8, 3, r, stats=True)
