@@ -88,6 +88,49 @@ def path_tracing_leg(ctx, cpu=True):
     return out
 
 
+def full_scene_leg(ctx, cpu=True):
+    """The full-feature scene (the room with the checker floor, three smooth trefoil
+    instances in gold, glass and copper, a Perlin marble sphere) at 1920x1080,
+    256 spp, max depth 16, roulette from bounce 3: one render, after a small one
+    that captures the bounce graph."""
+    from albedo_amd import workloads as W
+    from albedo_amd.scene import CameraSettings
+    s = W.full_scene()
+    g = ctx.scene(s)
+    c = W.ROOM_CAMERA
+    w, h, spa, depth = 1920, 1080, 16, 16
+    cam = CameraSettings(c.look_from, c.look_at, (0, 1, 0), 38.0, w / h)
+    g.render(cam, 192, 108, 2, depth, 3, 0)
+    t0 = time.perf_counter()
+    _, st = g.render(cam, w, h, spa, depth, 3, 1, stats=True)
+    wall = time.perf_counter() - t0
+    rays = st["camera_rays"] + st["extension_rays"] + st["shadow_rays"]
+    samples = w * h * spa * spa
+    out = {"workload": "full scene 1920x1080, 256 spp, max depth 16, roulette from 3",
+           "mpixel_spp_s": round(samples / wall / 1e6, 2), "mrays_s": round(rays / wall / 1e6, 2),
+           "s_per_image": round(wall, 3), "rays_per_image": int(rays), "gpu_ms": round(st["gpu_ms"], 1),
+           # The reference's README (README.md:74-75): meshes.scene, three trefoil instances on a
+           # checker floor, 1920x1080 at 256 spp in about 9 min on an 8-thread CPU, with its
+           # estimator's default max depth of 64 (this workload stops at 16).
+           "reference_published": {"mpixel_spp_s": 0.98, "s_per_image": 540.0,
+                                   "note": "meshes.scene, 8-thread CPU, max depth 64"}}
+    g.release()
+    print("full: rendered in %.2f s" % wall, file=sys.stderr, flush=True)
+    if cpu:
+        import oracle
+        o = oracle.OracleScene(s.to_desc())
+        cw, ch, cspa = 640, 360, 4
+        ccam = CameraSettings(c.look_from, c.look_at, (0, 1, 0), 38.0, cw / ch)
+        t0 = time.perf_counter()
+        _, counts = o.render(ccam.to_abi(), W.render_params(cw, ch, cspa, depth, 3, 1), sample_mode=0)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"mpixel_spp_s": round(cw * ch * cspa * cspa / dt / 1e6, 3),
+                               "mrays_s": round(float(counts.sum()) / dt / 1e6, 3), "cores": os.cpu_count(),
+                               "kind": "port",
+                               "sample": "the same scene at 640x360, 16 spp, FP64 oracle estimator, reference draw order"}
+    return out
+
+
 def batch_leg(ctx, n_jobs=16, full=False):
     """Mixed jobs (room / full-scene variants with 5..200 random spheres): one at a
     time vs one batched launch, each with its autotuned configuration. Default:
@@ -277,9 +320,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--trace-only", action="store_true", help="closest-hit benchmark only")
-    ap.add_argument("--legs", default="pt,batch,serve,build",
-                    help="legs after the closest-hit benchmark: any of pt, batch, serve, build, and the "
-                         "longer batch64 (64 jobs up to 1920x1080 at 256 spp), comma separated")
+    ap.add_argument("--legs", default="pt,full,batch,serve,build",
+                    help="legs after the closest-hit benchmark: any of pt, full, batch, serve, build, and "
+                         "the longer batch64 (64 jobs up to 1920x1080 at 256 spp), comma separated")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy, the reference for counter-based traffic")
     args = ap.parse_args()
@@ -372,6 +415,8 @@ def main():
     legs = set() if args.trace_only else set(args.legs.split(","))
     if "pt" in legs:
         out["path_tracing"] = path_tracing_leg(ctx, cpu=not args.no_cpu)
+    if "full" in legs:
+        out["full_scene"] = full_scene_leg(ctx, cpu=not args.no_cpu)
     if "batch" in legs:
         out["batched"] = batch_leg(ctx)
     if "batch64" in legs:
