@@ -406,9 +406,18 @@ ab_status Engine::renderBatch(const std::vector<JobSpec>& specs, const std::vect
     SceneSet set;
     for (const DevJob& j : jobs_) set.add(j.scene);
     chooseTracing(shape, set, capacity, true);
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0), "hipEventCreate");
-    HIP_TRY(hipEventCreate(&e1), "hipEventCreate");
+    // The timing events, destroyed on every way out of this function.
+    struct Events {
+        hipEvent_t e[2]{nullptr, nullptr};
+        ~Events()
+        {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } ev;
+    HIP_TRY(hipEventCreate(&ev.e[0]), "hipEventCreate");
+    HIP_TRY(hipEventCreate(&ev.e[1]), "hipEventCreate");
+    hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
     HIP_TRY(launchReset(ps_, total, uint32_t(jobs_.size()), stream_), "reset launch");
     HIP_TRY(hipEventRecord(e0, stream_), "hipEventRecord");
     Counters c{};
@@ -429,8 +438,6 @@ ab_status Engine::renderBatch(const std::vector<JobSpec>& specs, const std::vect
     (void)hipEventSynchronize(e1);
     float ms = 0;
     (void)hipEventElapsedTime(&ms, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     // On success the image copies above synchronised the stream; after a failure
     // the next render must not find work of this one still queued.
     if (st != AB_OK) (void)hipStreamSynchronize(stream_);
