@@ -193,6 +193,7 @@ ab_status ab_scene_create(ab_context* ctx, const ab_scene_desc* desc, ab_scene**
     s->dev.materials = reinterpret_cast<const DevMaterial*>(base + parts[6].off);
     s->dev.lights = reinterpret_cast<const DevLight*>(base + parts[7].off);
     s->dev.tlas_root = hs.tlas_root;
+    s->dev.num_objects = uint32_t(hs.objects.size());
     s->dev.num_lights = uint32_t(hs.lights.size());
     s->dev.sky_light = hs.sky_light;
     s->dev.stack_depth = stack;

@@ -59,7 +59,7 @@ struct alignas(16) DevObject {
     uint32_t mesh;
     uint32_t tri_shading_base; // first DevTriShading of the mesh
     uint32_t smooth;
-    uint32_t pad;
+    uint32_t tri_count;  // kept triangles of the mesh (instances): the bound of a hit's prim id
     float to_object[12]; // row-major 3x4, FP32
     float to_world[12];  // row-major 3x4, FP32
 };
@@ -108,6 +108,7 @@ struct DevScene {
     const DevMaterial* materials;
     const DevLight* lights;
     int32_t tlas_root;
+    uint32_t num_objects; // the bound of a hit's object id
     uint32_t num_lights;
     int32_t sky_light;
     uint32_t stack_depth; // most traversal-stack entries a query holds (traceRay and RayQuery)

@@ -281,6 +281,7 @@ BBox transformBox(const double* m, const BBox& b)
 struct MeshBuild {
     int32_t root{kNoChild};
     uint32_t shading_base{0};
+    uint32_t tri_count{0}; // kept triangles
     uint32_t depth{0};
     uint32_t nodes{0};
     BBox box;
@@ -467,6 +468,7 @@ ab_status buildHostScene(const ab_scene_desc& d, HostScene& out, std::string& er
         MeshBuild& b = mb[mi];
         b.smooth = m.smooth != 0;
         b.shading_base = uint32_t(out.shading.size());
+        b.tri_count = uint32_t(kept.size());
         const bool on_device = (d.build_flags & AB_BUILD_DEVICE_BLAS) && kept.size() > 4;
         // Unit vertex normals of a smooth mesh, once per vertex (none where the
         // sum has no length: those corners take the face normal).
@@ -634,6 +636,7 @@ ab_status buildHostScene(const ab_scene_desc& d, HostScene& out, std::string& er
             dobj.blas_root = b.root;
             dobj.mesh = o.mesh;
             dobj.tri_shading_base = b.shading_base;
+            dobj.tri_count = b.tri_count;
             dobj.smooth = b.smooth ? 1 : 0;
             for (int k = 0; k < 12; ++k) {
                 dobj.to_object[k] = float(o.to_object[k]);

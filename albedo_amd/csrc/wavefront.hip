@@ -112,6 +112,9 @@ __global__ void kGenerate(PathState ps)
         ps.prev[slot] = make_float4(ox, oy, oz, __uint_as_float(j));
         ps.rng[slot] = rng.state;
         ps.pixel[slot] = pix;
+        // No hit until the path is traced: a slot's previous path's hit must
+        // never reach shading (it may index another scene's arrays).
+        ps.hit[slot] = make_uint4(0x7f800000u, kMissId, kMissId, 0u);
         ps.active[act0 + i] = slot;
     }
 }
@@ -532,6 +535,13 @@ __device__ __forceinline__ uint32_t shadePath(const PathState& ps, uint32_t slot
         F3 thr = f3(t4.x, t4.y, t4.z), L = f3(r4.x, r4.y, r4.z);
         const int depth = __float_as_int(t4.w);
         const uint4 h = ps.hit[slot];
+        // Ids out of the scene's range (a hit that was never written for this
+        // path) count as a miss rather than index past the scene's arrays.
+        bool miss = h.y >= sc.num_objects;
+        if (!miss) {
+            const DevObject& ho = sc.objects[h.y];
+            miss = ho.kind != kObjSphere && h.z >= ho.tri_count;
+        }
         const float nl = float(sc.num_lights);
         // Estimators (reference estimator/radiance_estimator.hpp:36-38): the path
         // estimator weighs emission by MIS and adds next-event estimation and
@@ -543,7 +553,7 @@ __device__ __forceinline__ uint32_t shadePath(const PathState& ps, uint32_t slot
         bool alive = false, shadow = false;
         if (job.cancelled) {
             // terminated without gathering
-        } else if (h.y == kMissId) {
+        } else if (miss) {
             if (est == AB_ESTIMATOR_NORMALS) {
                 // White straight down to (0.5, 0.7, 1) straight up, on the normalised direction.
                 const float up = 0.5f * (normalize(rd).y + 1.0f);

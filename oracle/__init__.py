@@ -80,6 +80,12 @@ def lib():
         L.or_ggx_density.argtypes = [_D, _DP]
         L.or_ggx_masking.restype = _D
         L.or_ggx_masking.argtypes = [_D, _DP]
+        L.or_ggx_vn_sample_many.argtypes = [_D, _DP, _U64, _P, _P]
+        L.or_ggx_vn_pdf_many.argtypes = [_D, _DP, _U64, _P, _P]
+        L.or_lobe_eval_many.argtypes = [ctypes.c_int, _DP, _DP, _U64, _P, _P, _P]
+        L.or_lobe_sample_many.restype = _U64
+        L.or_lobe_sample_many.argtypes = [ctypes.c_int, _DP, _DP, _U64, _P, _P, _P, _P, _P]
+        L.or_radiance_est.argtypes = [_P, _DP, _DP, _U64, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, _U64, _DP]
         _lib = L
     return _lib
 
@@ -144,3 +150,51 @@ class OracleScene:
         out = (ctypes.c_double * 3)()
         lib().or_radiance(self.h, dvec(*origin), dvec(*direction), seed, max_depth, roulette_start, count, out)
         return np.array(out[:])
+
+    def radiance_est(self, origin, direction, estimator, seed=17, max_depth=64, roulette_start=3, count=1):
+        """Mean of `count` estimates along one ray with the chosen estimator
+        (0 path, 1 scatter-only with max_depth as its scatter limit, 2 normals)."""
+        out = (ctypes.c_double * 3)()
+        lib().or_radiance_est(self.h, dvec(*origin), dvec(*direction), seed, estimator, max_depth, roulette_start,
+                              count, out)
+        return np.array(out[:])
+
+
+# ------------------------------------------------------------ array helpers
+
+def _vec3(v):
+    return dvec(*[float(x) for x in v])
+
+
+def ggx_vn_sample(alpha, wo, u):
+    """Visible normals sampled from u [n, 2]."""
+    u = np.ascontiguousarray(u, np.float64)
+    m = np.zeros((len(u), 3))
+    lib().or_ggx_vn_sample_many(alpha, _vec3(wo), len(u), u.ctypes.data, m.ctypes.data)
+    return m
+
+
+def ggx_vn_pdf(alpha, wo, m):
+    m = np.ascontiguousarray(m, np.float64)
+    pdf = np.zeros(len(m))
+    lib().or_ggx_vn_pdf_many(alpha, _vec3(wo), len(m), m.ctypes.data, pdf.ctypes.data)
+    return pdf
+
+
+def lobe_eval_many(kind, params, wo, wi):
+    """(f [n, 3], pdf [n]) of a lobe (kind 0 Lambert, 1 conductor, 2 dielectric) for wi [n, 3]."""
+    wi = np.ascontiguousarray(wi, np.float64)
+    f = np.zeros((len(wi), 3))
+    pdf = np.zeros(len(wi))
+    lib().or_lobe_eval_many(kind, dvec(*params), _vec3(wo), len(wi), wi.ctypes.data, f.ctypes.data, pdf.ctypes.data)
+    return f, pdf
+
+
+def lobe_sample_many(kind, params, wo, u):
+    """u [n, 3] = (choice, x, y) -> (wi [n, 3], weight [n, 3], pdf [n], kept [n] bool)."""
+    u = np.ascontiguousarray(u, np.float64)
+    n = len(u)
+    wi, wt, pdf, kept = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n), np.zeros(n, np.uint8)
+    lib().or_lobe_sample_many(kind, dvec(*params), _vec3(wo), n, u.ctypes.data, wi.ctypes.data, wt.ctypes.data,
+                              pdf.ctypes.data, kept.ctypes.data)
+    return wi, wt, pdf, kept.astype(bool)

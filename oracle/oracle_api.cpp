@@ -851,4 +851,88 @@ int or_lobe_sample(int kind, const double* p, const double* wo, double uc, doubl
 double or_ggx_density(double alpha, const double* m) { return Ggx{alpha}.density({m[0], m[1], m[2]}); }
 double or_ggx_masking(double alpha, const double* w) { return Ggx{alpha}.masking({w[0], w[1], w[2]}); }
 
+// Array forms for the distribution tests, which compare Monte Carlo means of
+// samples with midpoint-rule quadrature of the densities over 10^5-10^6 points
+// (reference tests/scattering/ggx_distribution_test.cpp:154-188,
+// conductor_lobe_test.cpp:121-161, dielectric_lobe_test.cpp:199-237).
+// u: n x 2 (or n x 3 with the choice draw first); vectors n x 3.
+void or_ggx_vn_sample_many(double alpha, const double* wo, uint64_t n, const double* u, double* m)
+{
+    const Ggx g{alpha};
+    const V3d w{wo[0], wo[1], wo[2]};
+    for (uint64_t i = 0; i < n; ++i) {
+        const V3d s = g.sampleVisibleNormal(w, u[2 * i], u[2 * i + 1]);
+        m[3 * i] = s.x;
+        m[3 * i + 1] = s.y;
+        m[3 * i + 2] = s.z;
+    }
+}
+void or_ggx_vn_pdf_many(double alpha, const double* wo, uint64_t n, const double* m, double* pdf)
+{
+    const Ggx g{alpha};
+    const V3d w{wo[0], wo[1], wo[2]};
+    for (uint64_t i = 0; i < n; ++i) pdf[i] = g.visibleNormalPdf(w, {m[3 * i], m[3 * i + 1], m[3 * i + 2]});
+}
+void or_lobe_eval_many(int kind, const double* p, const double* wo, uint64_t n, const double* wi, double* f,
+                       double* pdf)
+{
+    const Lobe l = makeLobe(kind, p);
+    const V3d w{wo[0], wo[1], wo[2]};
+    for (uint64_t i = 0; i < n; ++i) {
+        const V3d d{wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]};
+        const Rgb r = l.evaluate(w, d);
+        f[3 * i] = r.r;
+        f[3 * i + 1] = r.g;
+        f[3 * i + 2] = r.b;
+        pdf[i] = l.pdf(w, d);
+    }
+}
+// Returns the number of samples kept; kept[i] = 0 where sample() gave nothing.
+uint64_t or_lobe_sample_many(int kind, const double* p, const double* wo, uint64_t n, const double* u, double* wi,
+                             double* weight, double* pdf, uint8_t* kept)
+{
+    const Lobe l = makeLobe(kind, p);
+    const V3d w{wo[0], wo[1], wo[2]};
+    uint64_t k = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        const auto s = l.sample(w, u[3 * i], u[3 * i + 1], u[3 * i + 2]);
+        kept[i] = s ? 1 : 0;
+        if (!s) {
+            wi[3 * i] = wi[3 * i + 1] = wi[3 * i + 2] = 0;
+            weight[3 * i] = weight[3 * i + 1] = weight[3 * i + 2] = 0;
+            pdf[i] = 0;
+            continue;
+        }
+        ++k;
+        wi[3 * i] = s->direction.x;
+        wi[3 * i + 1] = s->direction.y;
+        wi[3 * i + 2] = s->direction.z;
+        weight[3 * i] = s->weight.r;
+        weight[3 * i + 1] = s->weight.g;
+        weight[3 * i + 2] = s->weight.b;
+        pdf[i] = s->pdf;
+    }
+    return k;
+}
+
+// or_radiance with the estimator chosen as in ab_render_params (AB_ESTIMATOR_*;
+// for scatter-only, max_depth is the scatter limit), the mean of `count`
+// estimates drawn one after another from the stream of pixel (0, 0) of a
+// one-pixel image (the reference's test generator, path_estimator_test.cpp:31-34).
+void or_radiance_est(const or_scene* s, const double* o, const double* d, uint64_t seed, uint32_t estimator,
+                     int max_depth, int rr_start, uint64_t count, double* rgb)
+{
+    Generator g = pixelGenerator(seed, 0, 0, 1);
+    ab_render_params p{};
+    p.max_depth = uint32_t(max_depth);
+    p.roulette_start = uint32_t(rr_start);
+    p.estimator = estimator;
+    RayCounts rc;
+    Rgb sum{};
+    for (uint64_t i = 0; i < count; ++i) sum = sum + estimate(*s->s, p, {o[0], o[1], o[2]}, {d[0], d[1], d[2]}, g, rc);
+    rgb[0] = sum.r / double(count);
+    rgb[1] = sum.g / double(count);
+    rgb[2] = sum.b / double(count);
+}
+
 } // extern "C"
