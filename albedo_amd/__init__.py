@@ -168,6 +168,23 @@ class GpuScene:
         return out
 
 
+    def render_sums(self, camera: CameraSettings, width, height, samples_per_axis, max_depth=64, roulette_start=3,
+                    seed=0, estimator=_abi.ESTIMATOR_PATH, into=None):
+        """Adds the render's per-pixel sums to `into` (uint64 [height, width, 4],
+        the wire format; zeros when None) and returns it (ab_render_sums: the
+        reference renderImage's accumulating form)."""
+        from .workloads import render_params
+        p = render_params(width, height, samples_per_axis, max_depth, roulette_start, seed, estimator)
+        cam = camera.to_abi()
+        if into is None:
+            into = np.zeros((height, width, 4), np.uint64)
+        if into.dtype != np.uint64 or not into.flags.c_contiguous or into.shape != (height, width, 4):
+            raise ValueError("render_sums needs a C-contiguous uint64 [height, width, 4] array")
+        self.ctx.check(self.ctx.lib.ab_render_sums(self.handle, ctypes.byref(cam), ctypes.byref(p), into.ctypes.data,
+                                                   None))
+        return into
+
+
 def hits_t(hits: np.ndarray) -> np.ndarray:
     """The t column of a hits array as float32."""
     return hits[:, 0].copy().view(np.float32)

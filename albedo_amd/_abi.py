@@ -143,6 +143,8 @@ PROTOTYPES = {
     "ab_trace_closest_host": (_S, [_P, _P, _P, ctypes.c_uint64]),
     "ab_trace_any_host": (_S, [_P, _P, _P, ctypes.c_uint64]),
     "ab_render": (_S, [_P, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams), _P, ctypes.POINTER(RenderStats)]),
+    "ab_render_sums": (_S, [_P, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams), _P,
+                            ctypes.POINTER(RenderStats)]),
     "ab_set_launch_config": (_S, [_P, ctypes.POINTER(LaunchConfig)]),
     "ab_get_launch_config": (_S, [_P, ctypes.POINTER(LaunchConfig)]),
     "ab_render_batch": (_S, [_P, ctypes.POINTER(JobDesc), ctypes.c_uint32, ctypes.POINTER(_P),

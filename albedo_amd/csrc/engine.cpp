@@ -330,7 +330,7 @@ ab_status Engine::runUntilDone(const LaunchShape& shape, bool use_graph, Counter
 }
 
 ab_status Engine::renderBatch(const std::vector<JobSpec>& specs, const std::vector<float*>& out,
-                              ab_render_stats* stats, const ab_launch_config& cfg)
+                              ab_render_stats* stats, const ab_launch_config& cfg, const std::vector<uint64_t*>* sums)
 {
     if (specs.empty()) return AB_OK;
     jobs_.assign(specs.size(), DevJob{});
@@ -423,9 +423,22 @@ ab_status Engine::renderBatch(const std::vector<JobSpec>& specs, const std::vect
     Counters c{};
     st = runUntilDone(shape, cfg.use_graph != 0, c);
     (void)hipEventRecord(e1, stream_);
+    std::vector<uint64_t> words; // sums mode: one job's accumulators, then added to the caller's
     for (size_t i = 0; st == AB_OK && i < jobs_.size(); ++i) {
         const DevJob& j = jobs_[i];
         const size_t npx = size_t(j.width) * j.height;
+        if (sums) {
+            words.resize(npx * 4);
+            hipError_t e = hipMemcpyAsync(words.data(), j.accum, npx * 32, hipMemcpyDeviceToHost, stream_);
+            if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+            if (e != hipSuccess) {
+                st = hipFail(e, "sums readback");
+                break;
+            }
+            uint64_t* dst = (*sums)[i];
+            for (size_t k = 0; k < npx * 4; ++k) dst[k] += words[k];
+            continue;
+        }
         float* mean = reinterpret_cast<float*>(reinterpret_cast<char*>(j.accum) + bufs[i].mean_offset);
         hipError_t e = launchResolve(j.accum, npx, mean, stream_);
         if (e == hipSuccess) e = hipMemcpyAsync(out[i], mean, npx * 12, hipMemcpyDeviceToHost, stream_);
@@ -601,6 +614,22 @@ ab_status ab_render(ab_scene* scene, const ab_camera* camera, const ab_render_pa
     }
     JobSpec js{scene, *camera, *params};
     return contextEngine(ctx).renderBatch({js}, {rgb_mean}, stats, cfg);
+}
+
+ab_status ab_render_sums(ab_scene* scene, const ab_camera* camera, const ab_render_params* params, uint64_t* sums,
+                         ab_render_stats* stats)
+{
+    if (!scene || !camera || !params || !sums) return fail(AB_INVALID_ARGUMENT, "ab_render_sums: null argument");
+    ab_context* ctx = scene->ctx;
+    std::lock_guard<std::mutex> lock(ctx->render_mu);
+    ab_launch_config cfg;
+    {
+        std::lock_guard<std::mutex> l2(ctx->mu);
+        cfg = ctx->config;
+    }
+    JobSpec js{scene, *camera, *params};
+    const std::vector<uint64_t*> dst{sums};
+    return contextEngine(ctx).renderBatch({js}, {nullptr}, stats, cfg, &dst);
 }
 
 ab_status ab_render_batch(ab_context* ctx, const ab_job_desc* jobs, uint32_t n_jobs, float* const* rgb_means,

@@ -69,9 +69,11 @@ public:
     ab_status reserve(uint32_t capacity, uint32_t max_jobs, hipStream_t stream);
 
     // Renders every job to completion together; out[i] receives job i's
-    // per-pixel mean (width*height*3 floats).
+    // per-pixel mean (width*height*3 floats), or, when `sums` is given, job i's
+    // per-pixel sums are added to sums[i] (width*height*4 words, the wire format
+    // of ab_job_poll_sums) and `out` is unused.
     ab_status renderBatch(const std::vector<JobSpec>& jobs, const std::vector<float*>& out, ab_render_stats* stats,
-                          const ab_launch_config& cfg);
+                          const ab_launch_config& cfg, const std::vector<uint64_t*>* sums = nullptr);
 
     // Serving interface: jobs are appended while the engine runs. The job table
     // holds `max_jobs` entries and is emptied by resetTable() once idle. A job's

@@ -584,7 +584,7 @@ private:
             lights.push_back(d);
         }
         ab_scene_desc desc{meshes.data(), uint32_t(meshes.size()), objs.data(), uint32_t(objs.size()),
-                           mats.data(), uint32_t(mats.size()), lights.data(), uint32_t(lights.size())};
+                           mats.data(), uint32_t(mats.size()), lights.data(), uint32_t(lights.size()), 0};
         ab_scene* s = nullptr;
         detail::check(ab_scene_create(detail::context(), &desc, &s));
         return s;
@@ -632,41 +632,86 @@ struct PathEstimator {
 };
 using RadianceEstimator = std::variant<NormalsEstimator, ScatterOnlyEstimator, PathEstimator>;
 
-// Per-pixel mean radiance (the reference keeps sums and counts; the device keeps
-// them as fixed-point sums and returns the means).
+// Radiance samples accumulated per pixel: a sum and a sample count each
+// (reference image/pixel_buffer.hpp:17-57, pixel_buffer.cpp:9-45). renderImage
+// adds the device's per-pixel sums, which are exact fixed-point integers
+// (ab_render_sums), so a pixel rendered in several calls averages exactly as
+// one ab_sums_merge of those renders does.
 class PixelBuffer {
 public:
+    static constexpr std::size_t max_pixel_count{std::size_t{1} << 28};
+
     PixelBuffer(int width, int height) : w_(width), h_(height)
     {
-        if (width <= 0 || height <= 0 || int64_t(width) * height > (int64_t(1) << 28))
-            throw std::invalid_argument("invalid image size");
-        mean_.assign(std::size_t(width) * height * 3, 0.0f);
+        if (!isValidSize(width, height))
+            throw std::invalid_argument("invalid image size " + std::to_string(width) + "x" + std::to_string(height) +
+                                        ": dimensions must be positive and the image at most " +
+                                        std::to_string(max_pixel_count) + " pixels");
+        pixels_.resize(std::size_t(width) * std::size_t(height));
     }
-    int width() const { return w_; }
-    int height() const { return h_; }
-    Rgb average(Point2i p) const
+    static constexpr bool isValidSize(int width, int height) noexcept
     {
-        const std::size_t i = (std::size_t(p.y) * w_ + p.x) * 3;
-        return {mean_[i], mean_[i + 1], mean_[i + 2]};
+        if (width <= 0 || height <= 0) return false;
+        return std::size_t(width) <= max_pixel_count / std::size_t(height);
     }
-    float* data() { return mean_.data(); }
-    const float* data() const { return mean_.data(); }
+    int width() const noexcept { return w_; }
+    int height() const noexcept { return h_; }
+    // Precondition: `pixel` lies inside the image.
+    void addSample(Point2i pixel, Rgb sample) noexcept
+    {
+        Accumulator& a = pixels_[index(pixel)];
+        a.sum.r += sample.r;
+        a.sum.g += sample.g;
+        a.sum.b += sample.b;
+        ++a.sample_count;
+    }
+    // The mean of the pixel's samples, or black if it has none.
+    Rgb average(Point2i pixel) const noexcept
+    {
+        const Accumulator& a = pixels_[index(pixel)];
+        if (a.sample_count == 0) return {};
+        const Real n = Real(a.sample_count);
+        return {a.sum.r / n, a.sum.g / n, a.sum.b / n};
+    }
+    // Adds per-pixel sums in the wire format (4 words per pixel, row-major):
+    // the fixed-point sums are exact multiples of 2^-AB_SUMS_FRACTION_BITS, so
+    // their FP64 values add exactly while below 2^(53-AB_SUMS_FRACTION_BITS).
+    void addSums(const uint64_t* words) noexcept
+    {
+        const Real scale = Real(uint64_t(1) << AB_SUMS_FRACTION_BITS);
+        for (std::size_t p = 0; p < pixels_.size(); ++p) {
+            const uint64_t* w = words + 4 * p;
+            Accumulator& a = pixels_[p];
+            a.sum.r += Real(w[0]) / scale;
+            a.sum.g += Real(w[1]) / scale;
+            a.sum.b += Real(w[2]) / scale;
+            a.sample_count += int64_t(w[3]);
+        }
+    }
 
 private:
+    struct Accumulator {
+        Rgb sum{};
+        int64_t sample_count{0};
+    };
+    std::size_t index(Point2i p) const noexcept { return std::size_t(p.y) * std::size_t(w_) + std::size_t(p.x); }
+
     int w_, h_;
-    std::vector<float> mean_;
+    std::vector<Accumulator> pixels_;
 };
 
+// reference render/render_image.hpp:17-32, with the same defaults.
 struct RenderJob {
     std::reference_wrapper<const PerspectiveCamera> camera;
     std::reference_wrapper<const Scene> scene;
     int samples_per_axis{1};
     std::uint64_t seed{0};
-    RadianceEstimator estimator{PathEstimator{}};
+    RadianceEstimator estimator{ScatterOnlyEstimator{}};
     int thread_count{1}; // accepted for source compatibility; the GPU ignores it
 };
 
-// Renders `job` into `image` (reference render/render_image.cpp:58-77).
+// Renders `job` into `image`, adding samples_per_axis^2 samples to every pixel
+// (reference render/render_image.cpp:58-77).
 inline void renderImage(const RenderJob& job, PixelBuffer& image)
 {
     if (job.samples_per_axis <= 0)
@@ -703,7 +748,9 @@ inline void renderImage(const RenderJob& job, PixelBuffer& image)
     p.height = uint32_t(image.height());
     p.samples_per_axis = uint32_t(job.samples_per_axis);
     p.seed = job.seed;
-    detail::check(ab_render(job.scene.get().device(), &cam, &p, image.data(), nullptr));
+    std::vector<uint64_t> sums(std::size_t(image.width()) * std::size_t(image.height()) * 4, 0);
+    detail::check(ab_render_sums(job.scene.get().device(), &cam, &p, sums.data(), nullptr));
+    image.addSums(sums.data());
 }
 
 inline int availableThreadCount() noexcept { return 1; }

@@ -270,6 +270,18 @@ ab_status ab_trace_any_host(ab_scene* scene, const ab_ray* rays, uint8_t* occlud
 ab_status ab_render(ab_scene* scene, const ab_camera* camera, const ab_render_params* params,
                     float* rgb_mean, ab_render_stats* stats);
 
+/* Renders one image and ADDS its per-pixel sums to `sums` (host buffer of
+ * width*height*4 words in the wire format of ab_job_poll_sums: red, green and
+ * blue radiance sums in 2^-AB_SUMS_FRACTION_BITS fixed point, then the sample
+ * count), as the reference's renderImage adds samples_per_axis^2 samples to
+ * every pixel of the caller's PixelBuffer (render/render_image.hpp:37-48,
+ * image/pixel_buffer.cpp:29-35). Rendering again with another seed into the
+ * same words accumulates more samples; ab_sums_resolve gives the means, equal
+ * to ab_render's for a single render. Words wrap modulo 2^64 only past about
+ * 680 000 samples of the largest radiance (1e5) in one pixel. */
+ab_status ab_render_sums(ab_scene* scene, const ab_camera* camera, const ab_render_params* params, uint64_t* sums,
+                         ab_render_stats* stats);
+
 /* Launch-shape knobs of the wavefront engine. Results never depend on them. */
 typedef struct ab_launch_config {
     uint32_t block_size;     /* threads per block for the queue kernels: 64, 128 or 256 */

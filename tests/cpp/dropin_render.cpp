@@ -1,13 +1,45 @@
 // Exercises the C++ drop-in API the way the reference's own callers use it
 // (apps/albedo/main.cpp:85-106): build a Scene, a PerspectiveCamera and a
-// RenderJob, call renderImage into a PixelBuffer. Prints the image mean and
-// checks the reference's argument errors. Exit status 0 on success.
+// RenderJob, call renderImage into a PixelBuffer. Prints the image mean, checks
+// the reference's argument errors, the default estimator and renderImage's
+// accumulation, and writes the first image's per-pixel means (FP32 RGB) to
+// argv[2] when given. Exit status 0 on success.
 #include "albedo_dropin.hpp"
 
 #include <cstdio>
+#include <cstring>
 #include <memory>
 
 using namespace albedo;
+
+namespace {
+
+ab_camera abiCamera(const CameraSettings& s)
+{
+    ab_camera c{};
+    const Real f[3] = {s.look_from.x, s.look_from.y, s.look_from.z}, a[3] = {s.look_at.x, s.look_at.y, s.look_at.z},
+               u[3] = {s.up.x, s.up.y, s.up.z};
+    for (int k = 0; k < 3; ++k) {
+        c.look_from[k] = f[k];
+        c.look_at[k] = a[k];
+        c.up[k] = u[k];
+    }
+    c.vfov_degrees = s.vertical_fov_degrees;
+    c.aspect = s.aspect_ratio;
+    return c;
+}
+
+bool sameImage(const PixelBuffer& a, const PixelBuffer& b)
+{
+    for (int y = 0; y < a.height(); ++y)
+        for (int x = 0; x < a.width(); ++x) {
+            const Rgb p = a.average({x, y}), q = b.average({x, y});
+            if (p.r != q.r || p.g != q.g || p.b != q.b) return false;
+        }
+    return true;
+}
+
+} // namespace
 
 int main(int argc, char** argv)
 {
@@ -48,6 +80,76 @@ int main(int argc, char** argv)
             sum += c.r + c.g + c.b;
         }
     std::printf("mean %.6f\n", sum / (3.0 * width * height));
+    if (argc > 2) {
+        std::vector<float> out(std::size_t(width) * height * 3);
+        for (int y = 0; y < height; ++y)
+            for (int x = 0; x < width; ++x) {
+                const Rgb c = image.average({x, y});
+                float* o = &out[(std::size_t(y) * width + x) * 3];
+                o[0] = float(c.r);
+                o[1] = float(c.g);
+                o[2] = float(c.b);
+            }
+        FILE* f = std::fopen(argv[2], "wb");
+        if (!f || std::fwrite(out.data(), sizeof(float), out.size(), f) != out.size()) return 1;
+        std::fclose(f);
+    }
+
+    // renderImage adds samples (reference render_image.hpp:37-48): two seeds
+    // rendered into one buffer average like the merged sums of the two renders.
+    PixelBuffer twice{width, height};
+    renderImage(RenderJob{camera, scene, 2, 11, PathEstimator{8, 3}, 1}, twice);
+    renderImage(RenderJob{camera, scene, 2, 12, PathEstimator{8, 3}, 1}, twice);
+    const std::size_t npx = std::size_t(width) * height;
+    std::vector<uint64_t> first(npx * 4, 0), second(npx * 4, 0);
+    const ab_camera cam = abiCamera(camera.settings());
+    ab_render_params rp{};
+    rp.width = uint32_t(width);
+    rp.height = uint32_t(height);
+    rp.samples_per_axis = 2;
+    rp.max_depth = 8;
+    rp.roulette_start = 3;
+    rp.estimator = AB_ESTIMATOR_PATH;
+    rp.seed = 11;
+    if (ab_render_sums(scene.device(), &cam, &rp, first.data(), nullptr) != AB_OK) return 1;
+    rp.seed = 12;
+    if (ab_render_sums(scene.device(), &cam, &rp, second.data(), nullptr) != AB_OK) return 1;
+    std::vector<float> merged(npx * 3);
+    if (ab_sums_merge(first.data(), second.data(), npx) != AB_OK ||
+        ab_sums_resolve(first.data(), npx, merged.data()) != AB_OK)
+        return 1;
+    int mismatches = 0;
+    for (int y = 0; y < height; ++y)
+        for (int x = 0; x < width; ++x) {
+            const Rgb c = twice.average({x, y});
+            const float* m = &merged[(std::size_t(y) * width + x) * 3];
+            const float got[3] = {float(c.r), float(c.g), float(c.b)};
+            if (std::memcmp(got, m, sizeof(got)) != 0 || first[(std::size_t(y) * width + x) * 4 + 3] != 8) ++mismatches;
+        }
+    std::printf("accumulate_mismatches %d\n", mismatches);
+    // addSample keeps the reference's sums and counts.
+    PixelBuffer manual{2, 1};
+    manual.addSample({0, 0}, Rgb{1.0, 2.0, 3.0});
+    manual.addSample({0, 0}, Rgb{3.0, -2.0, 0.5});
+    const Rgb avg = manual.average({0, 0}), none = manual.average({1, 0});
+    const bool samples_ok = avg.r == 2.0 && avg.g == 0.0 && avg.b == 1.75 && none.r == 0 && none.g == 0 && none.b == 0;
+    std::printf("add_sample_ok %d\n", samples_ok ? 1 : 0);
+    // A default RenderJob scatters only, as the reference's does (render_image.hpp:28).
+    PixelBuffer by_default{width, height}, scatter_only{width, height};
+    renderImage(RenderJob{camera, scene, 2, 5}, by_default);
+    renderImage(RenderJob{camera, scene, 2, 5, ScatterOnlyEstimator{}}, scatter_only);
+    PixelBuffer path{width, height};
+    renderImage(RenderJob{camera, scene, 2, 5, PathEstimator{}}, path);
+    std::printf("default_is_scatter_only %d\n", sameImage(by_default, scatter_only) && !sameImage(by_default, path));
+    int size_errors = 0;
+    for (const auto& wh : {std::pair<int, int>{0, 4}, {4, -1}, {1 << 15, 1 << 14}})
+        try {
+            PixelBuffer bad{wh.first, wh.second};
+        } catch (const std::invalid_argument&) {
+            ++size_errors;
+        }
+    std::printf("size_errors %d\n", size_errors);
+    if (!PixelBuffer::isValidSize(1 << 14, 1 << 14) || PixelBuffer::isValidSize(1 << 14, (1 << 14) + 1)) return 1;
 
     // The other estimators of the reference's RadianceEstimator variant.
     PixelBuffer normals{width, height};
@@ -84,5 +186,5 @@ int main(int argc, char** argv)
         ++errors_seen;
     }
     std::printf("errors %d\n", errors_seen);
-    return (errors_seen == 3 && sum > 0) ? 0 : 1;
+    return (errors_seen == 3 && sum > 0 && mismatches == 0 && samples_ok && size_errors == 3) ? 0 : 1;
 }

@@ -26,10 +26,7 @@ static void expect(bool ok, const char* what)
 
 static void setPixel(PixelBuffer& image, int x, int y, float r, float g, float b)
 {
-    float* p = image.data() + (std::size_t(y) * image.width() + x) * 3;
-    p[0] = r;
-    p[1] = g;
-    p[2] = b;
+    image.addSample({x, y}, Rgb{r, g, b});
 }
 
 static PixelBuffer twoPixelImage()
