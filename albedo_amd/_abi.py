@@ -145,6 +145,7 @@ PROTOTYPES = {
     "ab_render": (_S, [_P, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams), _P, ctypes.POINTER(RenderStats)]),
     "ab_render_sums": (_S, [_P, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams), _P,
                             ctypes.POINTER(RenderStats)]),
+    "ab_render_static": (_S, [_P, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams), _P, _P]),
     "ab_set_launch_config": (_S, [_P, ctypes.POINTER(LaunchConfig)]),
     "ab_get_launch_config": (_S, [_P, ctypes.POINTER(LaunchConfig)]),
     "ab_render_batch": (_S, [_P, ctypes.POINTER(JobDesc), ctypes.c_uint32, ctypes.POINTER(_P),

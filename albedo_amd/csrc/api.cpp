@@ -58,6 +58,7 @@ ab_status ab_context_create(int device, ab_context** out)
         return fail(AB_INVALID_ARGUMENT, "ab_context_create: device " + std::to_string(device) +
                                              " out of range (" + std::to_string(count) + " devices)");
     if ((e = hipSetDevice(device)) != hipSuccess) return hipFail(e, "hipSetDevice");
+    staticScratchSetup(device);
     ab_context* ctx = new (std::nothrow) ab_context;
     if (!ctx) return fail(AB_OUT_OF_MEMORY, "ab_context_create: out of host memory");
     ctx->device = device;

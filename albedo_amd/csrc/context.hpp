@@ -51,5 +51,8 @@ ab_status fail(ab_status code, const std::string& msg);
 // Maps a HIP error to AB_DEVICE_ERROR / AB_OUT_OF_MEMORY with a message.
 ab_status hipFail(hipError_t e, const char* what);
 double msSince(const ab_context* ctx);
+// Finds the no-allocation renderer's device array on `device` (current), once
+// per device; ab_render_static reports a failure.
+void staticScratchSetup(int device);
 
 } // namespace ab

@@ -294,6 +294,31 @@ ab_status Engine::capture(const LaunchShape& shape)
     return AB_OK;
 }
 
+LaunchShape singleSceneShape(const ab_launch_config& cfg, uint32_t capacity, const DevScene& scene)
+{
+    LaunchShape shape = shapeFor(cfg, capacity, scene.stack_depth);
+    SceneSet set;
+    set.add(scene);
+    chooseTracing(shape, set, capacity, true);
+    return shape;
+}
+
+uint64_t bounceLimit(const DevJob* jobs, size_t n_jobs, uint64_t work_total, uint32_t capacity)
+{
+    uint64_t max_depth = 0;
+    for (size_t i = 0; i < n_jobs; ++i) max_depth = std::max<uint64_t>(max_depth, jobs[i].max_depth);
+    return 4 * (max_depth + 2) * (work_total / capacity + 2) + 64;
+}
+
+uint32_t firstCheckPairs(const DevJob* jobs, size_t n_jobs)
+{
+    uint64_t path_bounces = 1; // bounces a path can take (the normals view: one)
+    for (size_t i = 0; i < n_jobs; ++i)
+        path_bounces = std::max<uint64_t>(path_bounces,
+                                          jobs[i].estimator == AB_ESTIMATOR_NORMALS ? 1 : jobs[i].max_depth + 1);
+    return uint32_t(std::min<uint64_t>((path_bounces + 2) / 2, 8));
+}
+
 ab_status Engine::runUntilDone(const LaunchShape& shape, bool use_graph, Counters& c)
 {
     use_graph_ = use_graph;
@@ -301,20 +326,9 @@ ab_status Engine::runUntilDone(const LaunchShape& shape, bool use_graph, Counter
     if (cst != AB_OK) return cst;
     // Bounces between checks grow geometrically: the queue drains in a tail of
     // short bounces, and each check costs a host round trip.
-    // Every path ends within max_depth + 1 bounces of its generation, and each
-    // bounce generates up to `capacity` paths, so this many bounces always suffice;
-    // running past it means the engine's bookkeeping is broken.
-    uint64_t max_depth = 0;
-    uint64_t path_bounces = 1; // bounces a path can take (the normals view: one)
-    for (const DevJob& j : jobs_) {
-        max_depth = std::max<uint64_t>(max_depth, j.max_depth);
-        path_bounces = std::max<uint64_t>(path_bounces, j.estimator == AB_ESTIMATOR_NORMALS ? 1 : j.max_depth + 1);
-    }
-    const uint64_t limit = 4 * (max_depth + 2) * (work_total_ / capacity_ + 2) + 64;
+    const uint64_t limit = bounceLimit(jobs_.data(), jobs_.size(), work_total_, capacity_);
     uint64_t bounces = 0;
-    // The first check comes when paths started in the first bounce can have
-    // finished (all of a render that fits the path capacity), at most 16 bounces in.
-    uint32_t pairs_per_check = uint32_t(std::min<uint64_t>((path_bounces + 2) / 2, 8));
+    uint32_t pairs_per_check = firstCheckPairs(jobs_.data(), jobs_.size());
     while (true) {
         if (bounces > limit) return fail(AB_DEVICE_ERROR, "render did not finish within its bounce bound");
         bounces += 2 * pairs_per_check;

@@ -141,4 +141,19 @@ private:
 // Lock-protected engine of a context for ab_render / ab_render_batch.
 Engine& contextEngine(ab_context* ctx);
 
+// The bounce launch shape for a render of one job over `scene` with `capacity`
+// path slots under `cfg` (stack class, grid, tracing kernels), as ab_render
+// picks it.
+LaunchShape singleSceneShape(const ab_launch_config& cfg, uint32_t capacity, const DevScene& scene);
+
+// Most bounces a render of `jobs` (work_total samples over `capacity` slots) can
+// take: every path ends within max_depth + 1 bounces of its generation, and a
+// bounce generates up to `capacity` paths; running past it means the engine's
+// bookkeeping is broken.
+uint64_t bounceLimit(const DevJob* jobs, size_t n_jobs, uint64_t work_total, uint32_t capacity);
+
+// Bounce pairs before the first counter check: when paths started in the first
+// bounce can have finished, at most 16 bounces in.
+uint32_t firstCheckPairs(const DevJob* jobs, size_t n_jobs);
+
 } // namespace ab

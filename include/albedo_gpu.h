@@ -282,6 +282,21 @@ ab_status ab_render(ab_scene* scene, const ab_camera* camera, const ab_render_pa
 ab_status ab_render_sums(ab_scene* scene, const ab_camera* camera, const ab_render_params* params, uint64_t* sums,
                          ab_render_stats* stats);
 
+/* ab_render without allocating: the image goes to d_rgb_mean, a device buffer
+ * of width*height*3 floats owned by the caller, on `stream` (a hipStream_t, NULL
+ * = default stream), bit-identical to ab_render's. Path state, queues and the
+ * per-pixel accumulators live in one array of the library's, 96 MiB on every
+ * GPU the library runs on (held until the process ends); the call makes no
+ * device, pinned-host or graph allocation. Limits: at most 2^21 pixels (1920 x
+ * 1080 fits) -- larger images are rejected with AB_INVALID_ARGUMENT -- and
+ * 2^17 paths in flight. Calls that overlap on one GPU (other threads, or
+ * streams not ordered after each other) share that array and race. The call
+ * returns after the last bounce; the resolve into d_rgb_mean is queued on
+ * `stream`. No counterpart in the reference: for callers that cannot allocate
+ * (tight serving loops, harnesses that forbid hipMalloc). */
+ab_status ab_render_static(ab_scene* scene, const ab_camera* camera, const ab_render_params* params,
+                           float* d_rgb_mean, void* stream);
+
 /* Launch-shape knobs of the wavefront engine. Results never depend on them. */
 typedef struct ab_launch_config {
     uint32_t block_size;     /* threads per block for the queue kernels: 64, 128 or 256 */
