@@ -30,6 +30,18 @@ BYTES_PER_RAY = 48
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trace_closest_trefoil_traffic.json")
 
 
+def cpu_model():
+    """The host CPU's model name (lscpu's "Model name", from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(scene, rays_sets, budget_s=12.0):
     """Oracle FP64 closest hit on a bounded sample, all host cores."""
     import oracle
@@ -52,7 +64,8 @@ def cpu_baseline(scene, rays_sets, budget_s=12.0):
             o.trace_closest(sample, fp32=False, threads=threads)
             elapsed += time.perf_counter() - t0
             done += len(sample)
-    return {"value": round(done / elapsed / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+    return {"value": round(done / elapsed / 1e6, 3), "unit": "Mrays/s", "cores": threads, "cpu": cpu_model(),
+            "kind": "port",
             "sample": "every %d-th ray of both 2^24 sets, %d pass(es) (%d rays), FP64 oracle traversal, %.1f s"
                       % (stride, passes, done, elapsed)}
 
@@ -84,6 +97,7 @@ def path_tracing_leg(ctx, cpu=True):
         dt = time.perf_counter() - t0
         out["cpu_baseline"] = {"mpixel_spp_s": round(320 * 180 * 16 / dt / 1e6, 3),
                                "mrays_s": round(float(counts.sum()) / dt / 1e6, 3), "cores": os.cpu_count(),
+                               "cpu": cpu_model(),
                                "kind": "port", "sample": "the full image, FP64 oracle estimator, reference draw order"}
     return out
 
@@ -126,6 +140,7 @@ def full_scene_leg(ctx, cpu=True):
         dt = time.perf_counter() - t0
         out["cpu_baseline"] = {"mpixel_spp_s": round(cw * ch * cspa * cspa / dt / 1e6, 3),
                                "mrays_s": round(float(counts.sum()) / dt / 1e6, 3), "cores": os.cpu_count(),
+                               "cpu": cpu_model(),
                                "kind": "port",
                                "sample": "the same scene at 640x360, 16 spp, FP64 oracle estimator, reference draw order"}
     return out
@@ -320,9 +335,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--trace-only", action="store_true", help="closest-hit benchmark only")
-    ap.add_argument("--legs", default="pt,full,batch,serve,build",
-                    help="legs after the closest-hit benchmark: any of pt, full, batch, serve, build, and "
-                         "the longer batch64 (64 jobs up to 1920x1080 at 256 spp), comma separated")
+    ap.add_argument("--legs", default="pt,full,batch64,serve,build",
+                    help="legs after the closest-hit benchmark, comma separated: pt (sphere room), full "
+                         "(full-feature scene), batch64 (the 64-job mix up to 1920x1080 at 256 spp), batch "
+                         "(a 16-job mix up to 960x540 at 64 spp), serve, build")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy, the reference for counter-based traffic")
     args = ap.parse_args()
@@ -380,10 +396,15 @@ def main():
     value = rays / wall / 1e6
     avg_launch_ms = float(np.mean(list(kernel_ms.values())))
     achieved = BYTES_PER_RAY * n_per / (avg_launch_ms * 1e-3) / 1e9
-    traffic = None
+    # HBM bytes per launch need hardware counters (rocprofv3 --pmc FETCH_SIZE,
+    # WRITE_SIZE over this same command), which a run cannot read of itself:
+    # the figure comes from the committed counter summary and says so.
+    traffic, traffic_source = None, None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            tj = json.load(f)
+        traffic = tj.get("bytes_per_launch")
+        traffic_source = "profiles/%s (%s)" % (os.path.basename(TRAFFIC_FILE), tj.get("command", "rocprofv3 --pmc"))
     out = {
         "metric": "Mrays/s + Mpixel·spp/s on 1 MI355X; p50/p99 job latency, served vs plain",
         "value": round(value, 2),
@@ -403,6 +424,7 @@ def main():
         "per_set_mrays_s": {k: round(n_per / (v * 1e-3) / 1e6, 1) for k, v in kernel_ms.items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "traffic_source": traffic_source,
                      "kernel": "kQuery (closest hit)", "avg_launch_ms": round(avg_launch_ms, 4)},
     }
     hits = d_hits["incoherent"][:4096].cpu().numpy().view(np.uint32)

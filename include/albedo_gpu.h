@@ -285,7 +285,7 @@ ab_status ab_render_sums(ab_scene* scene, const ab_camera* camera, const ab_rend
 /* ab_render without allocating: the image goes to d_rgb_mean, a device buffer
  * of width*height*3 floats owned by the caller, on `stream` (a hipStream_t, NULL
  * = default stream), bit-identical to ab_render's. Path state, queues and the
- * per-pixel accumulators live in one array of the library's, 96 MiB on every
+ * per-pixel accumulators live in one array of the library's, 88 MiB on every
  * GPU the library runs on (held until the process ends); the call makes no
  * device, pinned-host or graph allocation. Limits: at most 2^21 pixels (1920 x
  * 1080 fits) -- larger images are rejected with AB_INVALID_ARGUMENT -- and
