@@ -12,10 +12,13 @@
 
 #include "albedo_dropin.hpp"
 
+#include <algorithm>
+#include <array>
 #include <charconv>
 #include <filesystem>
 #include <fstream>
 #include <istream>
+#include <limits>
 #include <map>
 #include <sstream>
 #include <string_view>
@@ -105,37 +108,80 @@ inline bool isIdentifier(std::string_view s)
     return true;
 }
 
-// Lines of at most `limit` bytes (without the terminator), numbered from 1.
+} // namespace detail
+
+// A line over a reader's length limit (reference io/line_reader.hpp:14-31).
+class LineTooLongError : public std::runtime_error {
+public:
+    LineTooLongError(std::size_t line, std::size_t limit, std::string_view format)
+        : std::runtime_error(std::string(format) + " line " + std::to_string(line) + " is longer than " +
+                             std::to_string(limit) + " bytes"),
+          line_(line), limit_(limit)
+    {
+    }
+    std::size_t line() const noexcept { return line_; }
+    std::size_t limit() const noexcept { return limit_; }
+
+private:
+    std::size_t line_, limit_;
+};
+
+namespace detail {
+
+// Lines of text with a bounded length, numbered from 1 (the rules of reference
+// io/line_reader.cpp): a line ends at '\n' or at the end of the input, one '\r'
+// before the end is dropped, and what remains may hold at most `limit` bytes, so
+// no more than limit + 1 bytes are ever held. `format` names the input in
+// messages ("OBJ", "scene").
 class LineReader {
 public:
-    LineReader(std::istream& in, std::size_t limit) : in_(in), limit_(limit) {}
+    LineReader(std::istream& in, std::size_t limit, std::string_view format)
+        : in_(in), limit_(limit), format_(format)
+    {
+        if (in.fail()) throw std::runtime_error(format_ + " input stream is not readable");
+        if (limit > std::size_t(std::numeric_limits<std::streamsize>::max()) - 2)
+            throw std::invalid_argument(format_ + " line length limit is too large");
+    }
+    // The next line into `line`; false at the end of the input.
     bool next(std::string& line)
     {
         line.clear();
-        int c;
-        bool any = false;
-        while ((c = in_.get()) != EOF) {
+        if (done_) return false;
+        bool any = false, newline = false;
+        for (int c; (c = in_.get()) != std::char_traits<char>::eof();) {
             any = true;
-            if (c == '\n') break;
-            if (line.size() >= limit_) {
-                ++number_;
-                throw std::length_error(std::to_string(number_));
+            if (c == '\n') {
+                newline = true;
+                break;
             }
+            if (line.size() > limit_) tooLong();
             line.push_back(char(c));
         }
-        if (!any) {
-            if (in_.bad()) throw std::runtime_error("cannot read input");
-            return false;
+        if (in_.bad()) {
+            done_ = true;
+            throw std::runtime_error("failed to read " + format_ + " input");
         }
+        if (!newline) done_ = true;
+        if (!any) return false;
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        if (line.size() > limit_) tooLong();
         ++number_;
         return true;
     }
     std::size_t number() const { return number_; }
 
 private:
+    [[noreturn]] void tooLong()
+    {
+        done_ = true;
+        throw LineTooLongError(number_ + 1, limit_, format_);
+    }
+
     std::istream& in_;
     std::size_t limit_;
+    std::string format_;
     std::size_t number_{0};
+    bool done_{false};
 };
 
 } // namespace detail
@@ -165,7 +211,7 @@ private:
 // indices from the end, optional texture indices); other statements ignored.
 inline TriangleMesh readObj(std::istream& input, const ObjLimits& limits = {})
 {
-    detail::LineReader lines(input, limits.max_line_length);
+    detail::LineReader lines(input, limits.max_line_length, "OBJ");
     std::vector<Point3r> pos;
     std::vector<Point2r> uv;
     std::vector<TriangleMesh::Face> faces, uv_faces;
@@ -257,11 +303,18 @@ inline TriangleMesh readObj(std::istream& input, const ObjLimits& limits = {})
                 }
             }
         }
-    } catch (const std::length_error& e) {
-        throw ObjParseError(std::stoull(e.what()),
-                            "line longer than " + std::to_string(limits.max_line_length) + " bytes");
+    } catch (const LineTooLongError& e) {
+        throw ObjParseError(e.line(), "line longer than " + std::to_string(e.limit()) + " bytes");
     }
     return TriangleMesh(std::move(pos), std::move(faces), std::move(uv), std::move(uv_faces));
+}
+
+// reference geometry/obj.cpp:251-263
+inline TriangleMesh readObj(const std::filesystem::path& path, const ObjLimits& limits = {})
+{
+    std::ifstream file(path, std::ios::binary);
+    if (!file) throw std::runtime_error("cannot open OBJ file '" + path.string() + "'");
+    return readObj(file, limits);
 }
 
 // ----------------------------------------------------------------- scenes
@@ -297,88 +350,126 @@ struct SceneDescription {
 
 namespace detail {
 
-constexpr Real kMaxReflectance = 1, kMaxEmission = 1e6, kMaxRoughness = 1, kMinIor = 1e-2, kMaxIor = 1e2,
-               kMinEta = 1e-3, kMaxEta = 1e3, kMaxK = 1e3, kMaxVeins = 1e2;
+// The scene format, as tables. A line is a statement: a keyword, then, by the
+// statement's row of kStatements, a leading argument and `key value...` fields
+// in any order (each at most once), described by rows of Field. Values are typed
+// by Arg; the texts that describe them and the messages a reader gives are the
+// reference's (scene/scene_file.cpp), kept here as data because callers match
+// on them.
+namespace text {
+inline std::string expected(const std::string& what) { return "expected " + what; }
+inline std::string notFinite(const std::string& what, std::string_view tok)
+{
+    return what + " must be a finite number, got " + quoted(tok);
+}
+inline std::string outOfRange(const std::string& what, Real lo, Real hi, Real v)
+{
+    return what + " must be between " + num(lo) + " and " + num(hi) + ", got " + num(v);
+}
+inline std::string notPositive(const std::string& what, Real v) { return what + " must be positive, got " + num(v); }
+constexpr const char* kAxes[3] = {"the x coordinate of ", "the y coordinate of ", "the z coordinate of "};
+constexpr const char* kChannels[3] = {"the red channel of ", "the green channel of ", "the blue channel of "};
+inline std::string badName(std::string_view tok) { return quoted(tok) + " is not a valid name"; }
+inline std::string twice(std::string_view key) { return quoted(key) + " is given twice"; }
+inline std::string unknownField(std::string_view key) { return "unknown field " + quoted(key); }
+inline std::string missing(std::string_view key) { return "missing '" + std::string(key) + "'"; }
+inline std::string unexpected(std::string_view tok) { return "unexpected " + quoted(tok); }
+inline std::string unknownStatement(std::string_view kw) { return "unknown statement " + quoted(kw); }
+inline std::string tooMany(std::size_t n, const char* things) { return "more than " + std::to_string(n) + " " + things; }
+inline std::string undefined(const char* kind, std::string_view name)
+{
+    return std::string(kind) + " " + quoted(name) + " is not defined";
+}
+inline std::string redefined(const char* kind, std::string_view name)
+{
+    return std::string(kind) + " " + quoted(name) + " is already defined";
+}
+inline std::string nestedMix(std::string_view name)
+{
+    return "material " + quoted(name) + " is a mix, and a mix cannot hold another mix";
+}
+inline std::string unknownMaterialType(std::string_view t) { return "unknown material type " + quoted(t); }
+inline std::string unknownOperation(std::string_view op) { return "unknown instance operation " + quoted(op); }
+inline std::string meshPath(std::string_view p, const char* problem) { return "mesh path " + quoted(p) + problem; }
+constexpr const char* kNoCamera = "the scene has no camera";
+constexpr const char* kSecondCamera = "the scene already has a camera";
+constexpr const char* kSecondSky = "the scene already has a sky";
+constexpr const char* kFov = "vertical field of view must be in (0, 180) degrees, got ";
+constexpr const char* kCameraDistance =
+    "distance from camera position to target must be non-zero, finite and within roughly [1e-154, 1e154]";
+constexpr const char* kCameraUp =
+    "length of the camera up direction must be non-zero, finite and within roughly [1e-154, 1e154]";
+constexpr const char* kCameraParallel = "camera up direction must not be parallel to the view";
+constexpr const char* kConductorChannel = "a conductor channel cannot have eta 1 and k 0";
+constexpr const char* kMeshNeedsFile = "meshes can only be read from a scene file, which they sit beside";
+constexpr const char* kMaterialTwice = "'material' is given twice";
+constexpr const char* kOutOfRangeBeforeFit = "the instance is out of range before 'fit'";
+constexpr const char* kCannotFit = "the instance is too large or too small to fit";
+constexpr const char* kOutOfRangeBeforeGround = "the instance is out of range before 'ground'";
+constexpr const char* kInstanceOutOfRange = "mesh instance transform takes the mesh out of range";
+} // namespace text
 
-class Statement {
+// Value ranges of the reference's reader: reflectances in [0, 1], emission up to
+// 1e6, roughness in [0, 1], dielectric index in [1e-2, 1e2], conductor eta in
+// [1e-3, 1e3] and k in [0, 1e3], Perlin veins in [0, 100].
+constexpr Real kReflectMax = 1, kEmitMax = 1e6, kRoughMax = 1, kIorMin = 1e-2, kIorMax = 1e2, kEtaMin = 1e-3,
+               kEtaMax = 1e3, kKMax = 1e3, kVeinsMax = 1e2;
+
+// How a value is written: three numbers (a point, a vector, a colour whose
+// channels lie in [lo, hi]), one number (any, positive, or in [lo, hi]), or the
+// name of a material defined earlier (Basic: one that is not a mix).
+enum class Arg : uint8_t { Point, Vector, Color, Number, Positive, Ranged, Material, Basic };
+
+struct Field {
+    std::string_view key;
+    Arg arg;
+    const char* what; // how messages name the value
+    Real lo{0}, hi{0};
+};
+
+// A field's value once read.
+struct Slot {
+    bool given{false};
+    Real v[3]{0, 0, 0};
+    const Material* material{nullptr};
+    Point3r point() const { return {v[0], v[1], v[2]}; }
+    Vector3r vector() const { return {v[0], v[1], v[2]}; }
+    Rgb color() const { return {v[0], v[1], v[2]}; }
+};
+
+// The tokens of one statement and the line it stands on.
+class Cursor {
 public:
-    Statement(std::string_view text, std::size_t line) : tokens_(text), line_(line) {}
+    Cursor(std::string_view text, std::size_t line) : tokens_(text), line_(line) {}
     [[noreturn]] void fail(const std::string& problem) const { throw SceneParseError(line_, problem); }
     std::optional<std::string_view> next() { return tokens_.next(); }
     std::optional<std::string_view> peek() const
     {
-        Tokens rest = tokens_;
-        return rest.next();
+        Tokens copy = tokens_;
+        return copy.next();
     }
-    std::string_view word(const std::string& what)
+    std::string_view take(const std::string& what)
     {
         const auto t = next();
-        if (!t) fail("expected " + what);
+        if (!t) fail(text::expected(what));
         return *t;
     }
-    Real number(const std::string& what)
+    Real real(const std::string& what)
     {
-        const std::string_view t = word(what);
+        const std::string_view t = take(what);
         const auto v = parseReal(t);
-        if (!v) fail(what + " must be a finite number, got " + quoted(t));
+        if (!v) fail(text::notFinite(what, t));
         return *v;
     }
-    Real inRange(const std::string& what, Real lo, Real hi)
+    std::string_view identifier(const std::string& what)
     {
-        const Real v = number(what);
-        if (v < lo || v > hi) fail(what + " must be between " + num(lo) + " and " + num(hi) + ", got " + num(v));
-        return v;
-    }
-    Real positive(const std::string& what)
-    {
-        const Real v = number(what);
-        if (!(v > 0)) fail(what + " must be positive, got " + num(v));
-        return v;
-    }
-    Point3r point(const std::string& what)
-    {
-        const Real x = number("the x coordinate of " + what);
-        const Real y = number("the y coordinate of " + what);
-        return {x, y, number("the z coordinate of " + what)};
-    }
-    Vector3r vector(const std::string& what)
-    {
-        const Point3r p = point(what);
-        return {p.x, p.y, p.z};
-    }
-    Rgb color(const std::string& what, Real lo, Real hi)
-    {
-        const Real r = inRange("the red channel of " + what, lo, hi);
-        const Real g = inRange("the green channel of " + what, lo, hi);
-        return {r, g, inRange("the blue channel of " + what, lo, hi)};
-    }
-    Rgb reflectance(const std::string& what) { return color(what, 0, kMaxReflectance); }
-    Rgb emission(const std::string& what) { return color(what, 0, kMaxEmission); }
-    std::string_view name(const std::string& what)
-    {
-        const std::string_view t = word(what);
-        if (!isIdentifier(t)) fail(quoted(t) + " is not a valid name");
+        const std::string_view t = take(what);
+        if (!isIdentifier(t)) fail(text::badName(t));
         return t;
     }
-    template <class Read>
-    void readFields(Read read)
+    void end()
     {
-        std::vector<std::string_view> seen;
-        while (const auto f = next()) {
-            if (std::find(seen.begin(), seen.end(), *f) != seen.end()) fail(quoted(*f) + " is given twice");
-            seen.push_back(*f);
-            if (!read(*f)) fail("unknown field " + quoted(*f));
-        }
-    }
-    template <class T>
-    T required(const std::optional<T>& v, const char* what) const
-    {
-        if (!v) fail(std::string("missing '") + what + "'");
-        return *v;
-    }
-    void expectEnd()
-    {
-        if (const auto extra = next()) fail("unexpected " + quoted(*extra));
+        if (const auto t = next()) fail(text::unexpected(*t));
     }
 
 private:
@@ -386,469 +477,521 @@ private:
     std::size_t line_;
 };
 
-inline bool isInstanceKeyword(std::string_view w)
+using MaterialTable = std::map<std::string, Material, std::less<>>;
+
+// Reads one value of kind `f.arg` into `s`; `materials` resolves names.
+inline void readValue(Cursor& c, const Field& f, Slot& s, const MaterialTable& materials)
 {
-    for (const char* k : {"material", "translate", "scale", "rotate", "fit", "ground"})
-        if (w == k) return true;
-    return false;
+    const std::string what = f.what;
+    const auto bounded = [&](const std::string& w) {
+        const Real v = c.real(w);
+        if (v < f.lo || v > f.hi) c.fail(text::outOfRange(w, f.lo, f.hi, v));
+        return v;
+    };
+    switch (f.arg) {
+    case Arg::Point:
+    case Arg::Vector:
+        for (int k = 0; k < 3; ++k) s.v[k] = c.real(text::kAxes[k] + what);
+        break;
+    case Arg::Color:
+        for (int k = 0; k < 3; ++k) s.v[k] = bounded(text::kChannels[k] + what);
+        break;
+    case Arg::Number:
+        s.v[0] = c.real(what);
+        break;
+    case Arg::Positive:
+        s.v[0] = c.real(what);
+        if (!(s.v[0] > 0)) c.fail(text::notPositive(what, s.v[0]));
+        break;
+    case Arg::Ranged:
+        s.v[0] = bounded(what);
+        break;
+    case Arg::Material:
+    case Arg::Basic: {
+        const std::string_view name = c.identifier("a material name");
+        const auto it = materials.find(name);
+        if (it == materials.end()) c.fail(text::undefined("material", name));
+        if (f.arg == Arg::Basic && std::holds_alternative<MixMaterial>(it->second)) c.fail(text::nestedMix(name));
+        s.material = &it->second;
+        break;
+    }
+    }
+    s.given = true;
 }
 
-struct LoadedMesh {
+// The fields of a statement, read to its end: each key at most once, in any order.
+template <std::size_t N>
+std::array<Slot, N> readFields(Cursor& c, const std::array<Field, N>& fields, const MaterialTable& materials)
+{
+    std::array<Slot, N> slots{};
+    while (const auto key = c.next()) {
+        std::size_t i = 0;
+        while (i < N && fields[i].key != *key) ++i;
+        if (i == N) c.fail(text::unknownField(*key));
+        if (slots[i].given) c.fail(text::twice(*key));
+        readValue(c, fields[i], slots[i], materials);
+    }
+    return slots;
+}
+
+// Slot `i` of a statement whose field table is `fields`; fails when it was not given.
+template <std::size_t N>
+const Slot& need(const Cursor& c, const std::array<Field, N>& fields, const std::array<Slot, N>& slots, std::size_t i)
+{
+    if (!slots[i].given) c.fail(text::missing(fields[i].key));
+    return slots[i];
+}
+
+inline Real orDefault(const Slot& s, Real value) { return s.given ? s.v[0] : value; }
+
+// ------------------------------------------------------------ field tables
+
+inline const std::array<Field, 4> kCameraFields{{{"from", Arg::Point, "the camera position"},
+                                                 {"at", Arg::Point, "the point the camera looks at"},
+                                                 {"up", Arg::Vector, "the camera's up direction"},
+                                                 {"fov", Arg::Number, "the field of view"}}};
+inline const std::array<Field, 2> kSkyFields{{{"nadir", Arg::Color, "the radiance from below", 0, kEmitMax},
+                                              {"zenith", Arg::Color, "the radiance from above", 0, kEmitMax}}};
+inline const Field kSkyUniform{"", Arg::Color, "the sky's radiance", 0, kEmitMax};
+inline const std::array<Field, 2> kSphereFields{{{"radius", Arg::Number, "a radius"},
+                                                 {"material", Arg::Material, "a material"}}};
+inline const std::array<Field, 1> kPointLightFields{{{"intensity", Arg::Color, "the intensity", 0, kEmitMax}}};
+inline const std::array<Field, 3> kAreaLightFields{{{"radius", Arg::Number, "a radius"},
+                                                    {"radiance", Arg::Color, "the radiance", 0, kEmitMax},
+                                                    {"material", Arg::Material, "a material"}}};
+inline const std::array<Field, 1> kDiffuseFields{{{"color", Arg::Color, "the color", 0, kReflectMax}}};
+inline const std::array<Field, 3> kCheckerFields{{{"even", Arg::Color, "the even color", 0, kReflectMax},
+                                                  {"odd", Arg::Color, "the odd color", 0, kReflectMax},
+                                                  {"frequency", Arg::Positive, "the checker frequency"}}};
+inline const std::array<Field, 4> kPerlinFields{{{"low", Arg::Color, "the low color", 0, kReflectMax},
+                                                 {"high", Arg::Color, "the high color", 0, kReflectMax},
+                                                 {"scale", Arg::Positive, "the noise scale"},
+                                                 {"veins", Arg::Ranged, "the veins", 0, kVeinsMax}}};
+inline const std::array<Field, 3> kConductorFields{{{"eta", Arg::Color, "the refractive index", kEtaMin, kEtaMax},
+                                                    {"k", Arg::Color, "the absorption", 0, kKMax},
+                                                    {"roughness", Arg::Ranged, "the roughness", 0, kRoughMax}}};
+inline const std::array<Field, 2> kDielectricFields{
+    {{"ior", Arg::Ranged, "the index of refraction", kIorMin, kIorMax},
+     {"roughness", Arg::Ranged, "the roughness", 0, kRoughMax}}};
+inline const std::array<Field, 5> kMixFields{{{"first", Arg::Basic, "a material"},
+                                              {"second", Arg::Basic, "a material"},
+                                              {"scale", Arg::Positive, "the noise scale"},
+                                              {"veins", Arg::Ranged, "the veins", 0, kVeinsMax},
+                                              {"threshold", Arg::Ranged, "the threshold", 0, 1}}};
+
+inline BasicMaterial basicOf(const Material& m)
+{
+    if (const auto* d = std::get_if<DiffuseMaterial>(&m)) return *d;
+    if (const auto* k = std::get_if<ConductorMaterial>(&m)) return *k;
+    return std::get<DielectricMaterial>(m);
+}
+
+// Material types: name -> reads the type's fields and makes the material.
+struct MaterialType {
+    std::string_view name;
+    Material (*make)(Cursor&, const MaterialTable&);
+};
+
+inline const std::array<MaterialType, 6> kMaterialTypes{{
+    {"diffuse",
+     [](Cursor& c, const MaterialTable& t) -> Material {
+         const auto s = readFields(c, kDiffuseFields, t);
+         return DiffuseMaterial{ConstantTexture{need(c, kDiffuseFields, s, 0).color()}};
+     }},
+    {"checker",
+     [](Cursor& c, const MaterialTable& t) -> Material {
+         const auto s = readFields(c, kCheckerFields, t);
+         const Rgb even = need(c, kCheckerFields, s, 0).color(), odd = need(c, kCheckerFields, s, 1).color();
+         return DiffuseMaterial{CheckerTexture{even, odd, need(c, kCheckerFields, s, 2).v[0]}};
+     }},
+    {"perlin",
+     [](Cursor& c, const MaterialTable& t) -> Material {
+         const auto s = readFields(c, kPerlinFields, t);
+         const Rgb low = need(c, kPerlinFields, s, 0).color(), high = need(c, kPerlinFields, s, 1).color();
+         return DiffuseMaterial{PerlinTexture{low, high, need(c, kPerlinFields, s, 2).v[0], orDefault(s[3], 0)}};
+     }},
+    {"conductor",
+     [](Cursor& c, const MaterialTable& t) -> Material {
+         const auto s = readFields(c, kConductorFields, t);
+         const Slot& eta = need(c, kConductorFields, s, 0);
+         const Slot& k = need(c, kConductorFields, s, 1);
+         for (int ch = 0; ch < 3; ++ch)
+             if (eta.v[ch] == 1 && k.v[ch] == 0) c.fail(text::kConductorChannel);
+         return ConductorMaterial{eta.color(), k.color(), orDefault(s[2], 0)};
+     }},
+    {"dielectric",
+     [](Cursor& c, const MaterialTable& t) -> Material {
+         const auto s = readFields(c, kDielectricFields, t);
+         return DielectricMaterial{need(c, kDielectricFields, s, 0).v[0], orDefault(s[1], 0)};
+     }},
+    {"mix",
+     [](Cursor& c, const MaterialTable& t) -> Material {
+         const auto s = readFields(c, kMixFields, t);
+         const BasicMaterial first = basicOf(*need(c, kMixFields, s, 0).material);
+         const BasicMaterial second = basicOf(*need(c, kMixFields, s, 1).material);
+         const Real scale = need(c, kMixFields, s, 2).v[0];
+         return MixMaterial{first, second, PerlinPattern{scale, orDefault(s[3], 0)}, orDefault(s[4], 0.5)};
+     }},
+}};
+
+// reference camera/perspective_camera.cpp:24-110: the checks a camera must pass.
+inline void checkCamera(const Cursor& c, const CameraSettings& s)
+{
+    const Real fov = s.vertical_fov_degrees;
+    if (std::isnan(fov) || fov <= 0 || fov >= 180) c.fail(text::kFov + num(fov));
+    const Real b[3] = {s.look_from.x - s.look_at.x, s.look_from.y - s.look_at.y, s.look_from.z - s.look_at.z};
+    const Real up[3] = {s.up.x, s.up.y, s.up.z};
+    const Real bb = b[0] * b[0] + b[1] * b[1] + b[2] * b[2], uu = up[0] * up[0] + up[1] * up[1] + up[2] * up[2];
+    if (!std::isnormal(bb)) c.fail(text::kCameraDistance);
+    if (!std::isnormal(uu)) c.fail(text::kCameraUp);
+    Real w[3], u[3];
+    for (int k = 0; k < 3; ++k) {
+        w[k] = b[k] / std::sqrt(bb);
+        u[k] = up[k] / std::sqrt(uu);
+    }
+    Real r2 = 0;
+    for (int k = 0; k < 3; ++k) {
+        const Real x = u[(k + 1) % 3] * w[(k + 2) % 3] - u[(k + 2) % 3] * w[(k + 1) % 3];
+        r2 += x * x;
+    }
+    if (r2 < 1e-12) c.fail(text::kCameraParallel);
+}
+
+// The checks of reference scene_file.cpp's mesh path rules: relative, no '..',
+// no control bytes or Windows device names, inside the scene's directory after
+// links are resolved, an existing regular file. Returns the resolved file.
+inline std::filesystem::path sandboxedMeshFile(const Cursor& c, const std::filesystem::path& base,
+                                               std::string_view path)
+{
+    namespace fs = std::filesystem;
+    if (std::any_of(path.begin(), path.end(), [](char ch) {
+            const auto b = static_cast<unsigned char>(ch);
+            return b < 0x20 || b == 0x7F;
+        }))
+        c.fail(text::meshPath(path, " must be UTF-8 without control characters"));
+    const fs::path rel{std::string(path)};
+    const bool climbs = std::any_of(rel.begin(), rel.end(), [](const fs::path& part) { return part == ".."; });
+    if (climbs || rel.is_absolute() || rel.has_root_name() || rel.has_root_directory())
+        c.fail(text::meshPath(path, " must be relative to the scene and not use '..'"));
+    static constexpr std::string_view kDevices[] = {"CON", "PRN", "AUX", "NUL"};
+    for (const fs::path& part : rel) {
+        std::string stem = part.string();
+        stem.resize(std::min(stem.find_first_of(".:"), stem.size()));
+        while (!stem.empty() && stem.back() == ' ') stem.pop_back();
+        std::transform(stem.begin(), stem.end(), stem.begin(), [](char ch) {
+            return ch >= 'a' && ch <= 'z' ? char(ch - 32) : ch;
+        });
+        const bool named = std::find(std::begin(kDevices), std::end(kDevices), stem) != std::end(kDevices);
+        const bool port = stem.size() == 4 && (stem.compare(0, 3, "COM") == 0 || stem.compare(0, 3, "LPT") == 0) &&
+                          stem[3] >= '1' && stem[3] <= '9';
+        if (named || port) c.fail(text::meshPath(path, " names a device"));
+    }
+    std::error_code ec;
+    const fs::path root = fs::canonical(base, ec);
+    if (ec) c.fail("the scene's directory cannot be resolved");
+    const fs::path file = fs::weakly_canonical(root / rel, ec);
+    if (ec) c.fail("cannot resolve mesh path " + quoted(path));
+    const auto [r, f] = std::mismatch(root.begin(), root.end(), file.begin(), file.end());
+    (void)f;
+    if (r != root.end()) c.fail(text::meshPath(path, " leads outside the scene's directory"));
+    const fs::file_type type = fs::status(file, ec).type();
+    if (type == fs::file_type::not_found) c.fail("mesh file " + quoted(path) + " does not exist");
+    if (type != fs::file_type::regular) c.fail(text::meshPath(path, " is not a regular file"));
+    return file;
+}
+
+// State of one read: what has been defined so far and the budgets left.
+struct SceneState {
+    SceneLimits limits;
+    std::optional<std::filesystem::path> base;
+    ObjLimits mesh_budget;
+    std::size_t ground_left;
+    std::size_t objects{0};
+    std::map<std::string, std::shared_ptr<const MeshGeometry>, std::less<>> meshes;
+    MaterialTable materials;
+    Scene scene;
+    std::optional<CameraSettings> camera;
+    bool sky{false};
+
+    SceneState(const SceneLimits& l, std::optional<std::filesystem::path> b)
+        : limits(l), base(std::move(b)), mesh_budget(l.mesh_limits), ground_left(l.max_ground_points)
+    {
+    }
+    void countObject(const Cursor& c)
+    {
+        if (objects >= limits.max_objects) c.fail(text::tooMany(limits.max_objects, "objects"));
+        ++objects;
+    }
+};
+
+inline Sphere validSphere(const Cursor& c, Point3r center, Real radius)
+{
+    try {
+        return Sphere{center, radius};
+    } catch (const std::invalid_argument& e) {
+        c.fail(e.what());
+    }
+}
+
+inline bool finiteBox(const Point3r& lo, const Point3r& hi)
+{
+    for (const Real v : {lo.x, lo.y, lo.z, hi.x, hi.y, hi.z})
+        if (!std::isfinite(v)) return false;
+    return true;
+}
+
+// Words that end a one-factor `scale`: the instance statement's keywords.
+inline constexpr std::array<std::string_view, 6> kInstanceWords{"material", "translate", "scale",
+                                                                "rotate",   "fit",       "ground"};
+
+// Instance operations: each applies after the placement so far
+// (reference Transform composition, core/transform.hpp).
+struct Operation {
+    std::string_view name;
+    Transform (*apply)(Cursor&, SceneState&, const MeshGeometry&, const Transform&);
+};
+
+inline const std::array<Operation, 5> kOperations{{
+    {"translate",
+     [](Cursor& c, SceneState&, const MeshGeometry&, const Transform& p) {
+         Slot s;
+         readValue(c, {"", Arg::Vector, "the translation"}, s, {});
+         return Transform::translation(s.vector()) * p;
+     }},
+    {"scale",
+     [](Cursor& c, SceneState&, const MeshGeometry&, const Transform& p) {
+         // One factor, or three: a second number follows unless the statement
+         // ends or the next word is a keyword of the instance statement.
+         const Real sx = c.real("a scale factor");
+         const auto nxt = c.peek();
+         const bool uniform = !nxt || std::find(kInstanceWords.begin(), kInstanceWords.end(), *nxt) !=
+                                          kInstanceWords.end();
+         if (uniform) return Transform::scaling({sx, sx, sx}) * p;
+         const Real sy = c.real("the y scale factor");
+         return Transform::scaling({sx, sy, c.real("the z scale factor")}) * p;
+     }},
+    {"rotate",
+     [](Cursor& c, SceneState&, const MeshGeometry&, const Transform& p) {
+         Slot axis;
+         readValue(c, {"", Arg::Vector, "the rotation axis"}, axis, {});
+         return Transform::rotation(axis.vector(), c.real("the rotation angle")) * p;
+     }},
+    {"fit",
+     [](Cursor& c, SceneState&, const MeshGeometry& g, const Transform& p) {
+         // Scales the placed mesh so its largest box extent is `size`, centred
+         // on the origin.
+         Slot size;
+         readValue(c, {"", Arg::Positive, "the fitted size"}, size, {});
+         Point3r lo, hi;
+         p.applyBox(g.boundsMin(), g.boundsMax(), lo, hi);
+         if (!finiteBox(lo, hi)) c.fail(text::kOutOfRangeBeforeFit);
+         const Real extent = std::max({hi.x - lo.x, hi.y - lo.y, hi.z - lo.z});
+         const Real k = size.v[0] / extent;
+         if (!std::isfinite(k) || !(k > 0)) c.fail(text::kCannotFit);
+         const Vector3r to_origin{-(lo.x / 2 + hi.x / 2), -(lo.y / 2 + hi.y / 2), -(lo.z / 2 + hi.z / 2)};
+         return Transform::scaling({k, k, k}) * Transform::translation(to_origin) * p;
+     }},
+    {"ground",
+     [](Cursor& c, SceneState& st, const MeshGeometry& g, const Transform& p) {
+         // Lifts the placed mesh so its lowest used point rests on y = 0.
+         const auto& points = g.usedPoints();
+         if (points.size() > st.ground_left)
+             c.fail("'ground' would examine more than " + std::to_string(st.limits.max_ground_points) +
+                    " points in the scene");
+         st.ground_left -= points.size();
+         Real lowest = std::numeric_limits<Real>::infinity();
+         for (const Point3r& q : points) {
+             const Real y = p.apply(q).y;
+             if (!std::isfinite(y)) c.fail(text::kOutOfRangeBeforeGround);
+             lowest = std::min(lowest, y);
+         }
+         return Transform::translation({0, -lowest, 0}) * p;
+     }},
+}};
+
+// Statements: keyword -> reader.
+struct StatementKind {
+    std::string_view keyword;
+    void (*read)(Cursor&, SceneState&);
+};
+
+inline void readCameraStatement(Cursor& c, SceneState& st)
+{
+    if (st.camera) c.fail(text::kSecondCamera);
+    const auto s = readFields(c, kCameraFields, st.materials);
+    CameraSettings cam{};
+    cam.look_from = need(c, kCameraFields, s, 0).point();
+    cam.look_at = need(c, kCameraFields, s, 1).point();
+    if (s[2].given) cam.up = s[2].vector();
+    cam.vertical_fov_degrees = orDefault(s[3], cam.vertical_fov_degrees);
+    checkCamera(c, cam);
+    st.camera = cam;
+}
+
+inline void readSkyStatement(Cursor& c, SceneState& st)
+{
+    if (st.sky) c.fail(text::kSecondSky);
+    // Either one radiance for the whole sky, or `nadir` and `zenith` fields.
+    const auto first = c.peek();
+    if (first && (*first == kSkyFields[0].key || *first == kSkyFields[1].key)) {
+        const auto s = readFields(c, kSkyFields, st.materials);
+        const Rgb nadir = need(c, kSkyFields, s, 0).color();
+        st.scene.setSky(SkyLight{nadir, need(c, kSkyFields, s, 1).color()});
+    } else {
+        Slot s;
+        readValue(c, kSkyUniform, s, st.materials);
+        c.end();
+        st.scene.setSky(SkyLight::uniform(s.color()));
+    }
+    st.sky = true;
+}
+
+inline void readMaterialStatement(Cursor& c, SceneState& st)
+{
+    if (st.materials.size() >= st.limits.max_materials) c.fail(text::tooMany(st.limits.max_materials, "materials"));
+    const std::string_view name = c.identifier("a material name");
+    if (st.materials.count(name)) c.fail(text::redefined("material", name));
+    const std::string_view type = c.take("a material type");
+    const auto kind = std::find_if(kMaterialTypes.begin(), kMaterialTypes.end(),
+                                   [&](const MaterialType& m) { return m.name == type; });
+    if (kind == kMaterialTypes.end()) c.fail(text::unknownMaterialType(type));
+    Material m = kind->make(c, st.materials);
+    st.materials.emplace(std::string(name), std::move(m));
+}
+
+inline void readSphereStatement(Cursor& c, SceneState& st)
+{
+    st.countObject(c);
+    Slot center;
+    readValue(c, {"", Arg::Point, "the sphere center"}, center, st.materials);
+    const auto s = readFields(c, kSphereFields, st.materials);
+    const Sphere sphere = validSphere(c, center.point(), need(c, kSphereFields, s, 0).v[0]);
+    st.scene.add(sphere, *need(c, kSphereFields, s, 1).material);
+}
+
+inline void readPointLightStatement(Cursor& c, SceneState& st)
+{
+    st.countObject(c);
+    Slot at;
+    readValue(c, {"", Arg::Point, "the light position"}, at, st.materials);
+    const auto s = readFields(c, kPointLightFields, st.materials);
+    st.scene.addLight(PointLight{at.point(), need(c, kPointLightFields, s, 0).color()});
+}
+
+inline void readAreaLightStatement(Cursor& c, SceneState& st)
+{
+    st.countObject(c);
+    Slot center;
+    readValue(c, {"", Arg::Point, "the light center"}, center, st.materials);
+    const auto s = readFields(c, kAreaLightFields, st.materials);
+    const Sphere sphere = validSphere(c, center.point(), need(c, kAreaLightFields, s, 0).v[0]);
+    const Rgb radiance = need(c, kAreaLightFields, s, 1).color();
+    st.scene.addAreaLight(sphere, radiance, s[2].given ? *s[2].material : Material{DiffuseMaterial{}});
+}
+
+inline void readMeshStatement(Cursor& c, SceneState& st)
+{
+    if (!st.base) c.fail(text::kMeshNeedsFile);
+    if (st.meshes.size() >= st.limits.max_meshes) c.fail(text::tooMany(st.limits.max_meshes, "meshes"));
+    const std::string_view name = c.identifier("a mesh name");
+    if (st.meshes.count(name)) c.fail(text::redefined("mesh", name));
+    const std::string_view path = c.take("a mesh file");
+    const bool smooth = c.peek() == std::optional<std::string_view>{"smooth"};
+    if (smooth) (void)c.next();
+    c.end();
+    const std::filesystem::path file = sandboxedMeshFile(c, *st.base, path);
+    std::ifstream in(file, std::ios::binary);
+    if (!in) c.fail("cannot read mesh file " + quoted(path));
+    std::optional<TriangleMesh> mesh;
+    try {
+        mesh.emplace(readObj(in, st.mesh_budget));
+    } catch (const ObjParseError& e) {
+        c.fail("mesh " + quoted(path) + ": " + e.what());
+    } catch (const std::out_of_range& e) {
+        c.fail("mesh " + quoted(path) + ": " + e.what());
+    } catch (const std::runtime_error&) {
+        c.fail("cannot read mesh file " + quoted(path));
+    }
     std::shared_ptr<const MeshGeometry> geometry;
-};
-
-// The reader of scene/scene_file.cpp:443-1050.
-class SceneReader {
-public:
-    SceneReader(const SceneLimits& limits, std::optional<std::filesystem::path> base)
-        : limits_(limits), base_(std::move(base)), mesh_budget_(limits.mesh_limits),
-          ground_left_(limits.max_ground_points)
-    {
+    try {
+        geometry = std::make_shared<const MeshGeometry>(*mesh, smooth ? MeshShading::Smooth : MeshShading::Flat);
+    } catch (const std::invalid_argument&) {
+        c.fail("mesh " + quoted(path) + " has no triangles");
     }
+    // Meshes of one scene share the OBJ budgets.
+    st.mesh_budget.max_vertices -= mesh->positions().size();
+    st.mesh_budget.max_texture_coordinates -= mesh->textureCoordinates().size();
+    st.mesh_budget.max_triangles -= mesh->faces().size();
+    st.meshes.emplace(std::string(name), std::move(geometry));
+}
 
-    void readLine(std::string_view line, std::size_t number)
-    {
-        Statement st(line.substr(0, line.find('#')), number);
-        const auto kw = st.next();
-        if (!kw) return;
-        if (*kw == "camera") readCamera(st);
-        else if (*kw == "sky") readSky(st);
-        else if (*kw == "material") readMaterial(st);
-        else if (*kw == "sphere") readSphere(st);
-        else if (*kw == "point_light") readPointLight(st);
-        else if (*kw == "area_light") readAreaLight(st);
-        else if (*kw == "mesh") readMesh(st);
-        else if (*kw == "instance") readInstance(st);
-        else st.fail("unknown statement " + quoted(*kw));
-    }
-
-    SceneDescription finish()
-    {
-        if (!camera_) throw SceneParseError(0, "the scene has no camera");
-        return {std::move(scene_), *camera_};
-    }
-
-private:
-    void readCamera(Statement& st)
-    {
-        if (camera_) st.fail("the scene already has a camera");
-        std::optional<Point3r> from, at;
-        CameraSettings s{};
-        st.readFields([&](std::string_view f) {
-            if (f == "from") from = st.point("the camera position");
-            else if (f == "at") at = st.point("the point the camera looks at");
-            else if (f == "up") s.up = st.vector("the camera's up direction");
-            else if (f == "fov") s.vertical_fov_degrees = st.number("the field of view");
-            else return false;
-            return true;
-        });
-        s.look_from = st.required(from, "from");
-        s.look_at = st.required(at, "at");
-        checkCamera(st, s);
-        camera_ = s;
-    }
-
-    // reference camera/perspective_camera.cpp:24-110
-    static void checkCamera(const Statement& st, const CameraSettings& s)
-    {
-        const Real fov = s.vertical_fov_degrees;
-        if (std::isnan(fov) || fov <= 0 || fov >= 180)
-            st.fail("vertical field of view must be in (0, 180) degrees, got " + num(fov));
-        const Real b[3] = {s.look_from.x - s.look_at.x, s.look_from.y - s.look_at.y, s.look_from.z - s.look_at.z};
-        const Real bl2 = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
-        if (!std::isnormal(bl2))
-            st.fail("distance from camera position to target must be non-zero, finite and within roughly "
-                    "[1e-154, 1e154]");
-        const Real ul2 = s.up.x * s.up.x + s.up.y * s.up.y + s.up.z * s.up.z;
-        if (!std::isnormal(ul2))
-            st.fail("length of the camera up direction must be non-zero, finite and within roughly [1e-154, 1e154]");
-        const Real bl = std::sqrt(bl2), ul = std::sqrt(ul2);
-        const Real w[3] = {b[0] / bl, b[1] / bl, b[2] / bl}, u[3] = {s.up.x / ul, s.up.y / ul, s.up.z / ul};
-        const Real r[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
-        if (r[0] * r[0] + r[1] * r[1] + r[2] * r[2] < 1e-12)
-            st.fail("camera up direction must not be parallel to the view");
-    }
-
-    void readSky(Statement& st)
-    {
-        if (sky_given_) st.fail("the scene already has a sky");
-        const auto first = st.peek();
-        if (first && (*first == "nadir" || *first == "zenith")) {
-            std::optional<Rgb> nadir, zenith;
-            st.readFields([&](std::string_view f) {
-                if (f == "nadir") nadir = st.emission("the radiance from below");
-                else if (f == "zenith") zenith = st.emission("the radiance from above");
-                else return false;
-                return true;
-            });
-            scene_.setSky(SkyLight{st.required(nadir, "nadir"), st.required(zenith, "zenith")});
-        } else {
-            scene_.setSky(SkyLight::uniform(st.emission("the sky's radiance")));
-            st.expectEnd();
+inline void readInstanceStatement(Cursor& c, SceneState& st)
+{
+    st.countObject(c);
+    const std::string_view mesh_name = c.identifier("a mesh name");
+    const auto mesh = st.meshes.find(mesh_name);
+    if (mesh == st.meshes.end()) c.fail(text::undefined("mesh", mesh_name));
+    // `material NAME` anywhere, and placement operations applied in order.
+    Slot material;
+    Transform placement;
+    std::size_t ops = 0;
+    while (const auto word = c.next()) {
+        if (*word == "material") {
+            if (material.given) c.fail(text::kMaterialTwice);
+            readValue(c, {"material", Arg::Material, "a material"}, material, st.materials);
+            continue;
         }
-        sky_given_ = true;
-    }
-
-    void readMaterial(Statement& st)
-    {
-        if (materials_.size() >= limits_.max_materials)
-            st.fail("more than " + std::to_string(limits_.max_materials) + " materials");
-        const std::string_view name = st.name("a material name");
-        if (materials_.count(std::string(name))) st.fail("material " + quoted(name) + " is already defined");
-        const std::string_view type = st.word("a material type");
-        Material m = type == "mix" ? readMix(st) : ofType(st, type);
-        materials_.emplace(std::string(name), m);
-    }
-
-    static Material ofType(Statement& st, std::string_view type)
-    {
-        if (type == "diffuse") {
-            std::optional<Rgb> color;
-            st.readFields([&](std::string_view f) {
-                if (f != "color") return false;
-                color = st.reflectance("the color");
-                return true;
-            });
-            return DiffuseMaterial{ConstantTexture{st.required(color, "color")}};
-        }
-        if (type == "checker") {
-            std::optional<Rgb> even, odd;
-            std::optional<Real> freq;
-            st.readFields([&](std::string_view f) {
-                if (f == "even") even = st.reflectance("the even color");
-                else if (f == "odd") odd = st.reflectance("the odd color");
-                else if (f == "frequency") freq = st.positive("the checker frequency");
-                else return false;
-                return true;
-            });
-            return DiffuseMaterial{CheckerTexture{st.required(even, "even"), st.required(odd, "odd"),
-                                                  st.required(freq, "frequency")}};
-        }
-        if (type == "perlin") {
-            std::optional<Rgb> low, high;
-            std::optional<Real> scale;
-            Real veins = 0;
-            st.readFields([&](std::string_view f) {
-                if (f == "low") low = st.reflectance("the low color");
-                else if (f == "high") high = st.reflectance("the high color");
-                else if (f == "scale") scale = st.positive("the noise scale");
-                else if (f == "veins") veins = st.inRange("the veins", 0, kMaxVeins);
-                else return false;
-                return true;
-            });
-            return DiffuseMaterial{PerlinTexture{st.required(low, "low"), st.required(high, "high"),
-                                                 st.required(scale, "scale"), veins}};
-        }
-        if (type == "conductor") {
-            std::optional<Rgb> eta, k;
-            Real rough = 0;
-            st.readFields([&](std::string_view f) {
-                if (f == "eta") eta = st.color("the refractive index", kMinEta, kMaxEta);
-                else if (f == "k") k = st.color("the absorption", 0, kMaxK);
-                else if (f == "roughness") rough = st.inRange("the roughness", 0, kMaxRoughness);
-                else return false;
-                return true;
-            });
-            const ConductorMaterial m{st.required(eta, "eta"), st.required(k, "k"), rough};
-            const Real e[3] = {m.eta.r, m.eta.g, m.eta.b}, kk[3] = {m.k.r, m.k.g, m.k.b};
-            for (int c = 0; c < 3; ++c)
-                if (e[c] == 1 && kk[c] == 0) st.fail("a conductor channel cannot have eta 1 and k 0");
-            return m;
-        }
-        if (type == "dielectric") {
-            std::optional<Real> ior;
-            Real rough = 0;
-            st.readFields([&](std::string_view f) {
-                if (f == "ior") ior = st.inRange("the index of refraction", kMinIor, kMaxIor);
-                else if (f == "roughness") rough = st.inRange("the roughness", 0, kMaxRoughness);
-                else return false;
-                return true;
-            });
-            return DielectricMaterial{st.required(ior, "ior"), rough};
-        }
-        st.fail("unknown material type " + quoted(type));
-    }
-
-    Material readMix(Statement& st) const
-    {
-        std::optional<BasicMaterial> first, second;
-        std::optional<Real> scale;
-        Real veins = 0, threshold = 0.5;
-        st.readFields([&](std::string_view f) {
-            if (f == "first") first = basicNamed(st);
-            else if (f == "second") second = basicNamed(st);
-            else if (f == "scale") scale = st.positive("the noise scale");
-            else if (f == "veins") veins = st.inRange("the veins", 0, kMaxVeins);
-            else if (f == "threshold") threshold = st.inRange("the threshold", 0, 1);
-            else return false;
-            return true;
-        });
-        return MixMaterial{st.required(first, "first"), st.required(second, "second"),
-                           PerlinPattern{st.required(scale, "scale"), veins}, threshold};
-    }
-
-    BasicMaterial basicNamed(Statement& st) const
-    {
-        const std::string_view name = st.peek().value_or("");
-        const Material& m = materialNamed(st);
-        if (std::holds_alternative<MixMaterial>(m))
-            st.fail("material " + quoted(name) + " is a mix, and a mix cannot hold another mix");
-        if (const auto* d = std::get_if<DiffuseMaterial>(&m)) return *d;
-        if (const auto* c = std::get_if<ConductorMaterial>(&m)) return *c;
-        return std::get<DielectricMaterial>(m);
-    }
-
-    const Material& materialNamed(Statement& st) const
-    {
-        const std::string_view name = st.name("a material name");
-        const auto it = materials_.find(std::string(name));
-        if (it == materials_.end()) st.fail("material " + quoted(name) + " is not defined");
-        return it->second;
-    }
-
-    void countObject(const Statement& st)
-    {
-        if (objects_ >= limits_.max_objects) st.fail("more than " + std::to_string(limits_.max_objects) + " objects");
-        ++objects_;
-    }
-
-    static Sphere sphereAt(const Statement& st, Point3r c, Real r)
-    {
+        if (++ops > st.limits.max_instance_operations)
+            c.fail(text::tooMany(st.limits.max_instance_operations, "operations"));
+        const auto op = std::find_if(kOperations.begin(), kOperations.end(),
+                                     [&](const Operation& o) { return o.name == *word; });
+        if (op == kOperations.end()) c.fail(text::unknownOperation(*word));
         try {
-            return Sphere{c, r};
+            placement = op->apply(c, st, *mesh->second, placement);
         } catch (const std::invalid_argument& e) {
-            st.fail(e.what());
+            c.fail(e.what());
         }
     }
+    if (!material.given) c.fail(text::missing("material"));
+    Point3r lo, hi;
+    placement.applyBox(mesh->second->boundsMin(), mesh->second->boundsMax(), lo, hi);
+    if (!finiteBox(lo, hi)) c.fail(text::kInstanceOutOfRange);
+    st.scene.add(MeshInstance{mesh->second, placement}, *material.material);
+}
 
-    void readSphere(Statement& st)
-    {
-        countObject(st);
-        const Point3r c = st.point("the sphere center");
-        std::optional<Real> radius;
-        const Material* m = nullptr;
-        st.readFields([&](std::string_view f) {
-            if (f == "radius") radius = st.number("a radius");
-            else if (f == "material") m = &materialNamed(st);
-            else return false;
-            return true;
-        });
-        const Sphere s = sphereAt(st, c, st.required(radius, "radius"));
-        if (!m) st.fail("missing 'material'");
-        scene_.add(s, *m);
-    }
+inline const std::array<StatementKind, 8> kStatements{{{"camera", readCameraStatement},
+                                                       {"sky", readSkyStatement},
+                                                       {"material", readMaterialStatement},
+                                                       {"sphere", readSphereStatement},
+                                                       {"point_light", readPointLightStatement},
+                                                       {"area_light", readAreaLightStatement},
+                                                       {"mesh", readMeshStatement},
+                                                       {"instance", readInstanceStatement}}};
 
-    void readPointLight(Statement& st)
-    {
-        countObject(st);
-        const Point3r p = st.point("the light position");
-        std::optional<Rgb> intensity;
-        st.readFields([&](std::string_view f) {
-            if (f != "intensity") return false;
-            intensity = st.emission("the intensity");
-            return true;
-        });
-        scene_.addLight(PointLight{p, st.required(intensity, "intensity")});
-    }
-
-    void readAreaLight(Statement& st)
-    {
-        countObject(st);
-        const Point3r c = st.point("the light center");
-        std::optional<Real> radius;
-        std::optional<Rgb> radiance;
-        Material m{DiffuseMaterial{}};
-        st.readFields([&](std::string_view f) {
-            if (f == "radius") radius = st.number("a radius");
-            else if (f == "radiance") radiance = st.emission("the radiance");
-            else if (f == "material") m = materialNamed(st);
-            else return false;
-            return true;
-        });
-        const Sphere s = sphereAt(st, c, st.required(radius, "radius"));
-        scene_.addAreaLight(s, st.required(radiance, "radiance"), m);
-    }
-
-    // Path rules of reference scene_file.cpp relativeMeshPath/meshFile.
-    std::filesystem::path meshFile(const Statement& st, std::string_view text) const
-    {
-        for (char c : text) {
-            const auto b = static_cast<unsigned char>(c);
-            if (b < 0x20 || b == 0x7F) st.fail("mesh path " + quoted(text) + " must be UTF-8 without control characters");
-        }
-        const std::filesystem::path rel{std::string(text)};
-        bool dotdot = false;
-        for (const auto& part : rel)
-            if (part == "..") dotdot = true;
-        if (rel.is_absolute() || rel.has_root_name() || rel.has_root_directory() || dotdot)
-            st.fail("mesh path " + quoted(text) + " must be relative to the scene and not use '..'");
-        for (const auto& part : rel) {
-            std::string base = part.string();
-            base.erase(std::min(base.find_first_of(".:"), base.size()));
-            while (!base.empty() && base.back() == ' ') base.pop_back();
-            for (char& c : base)
-                if (c >= 'a' && c <= 'z') c = char(c - 'a' + 'A');
-            const bool plain = base == "CON" || base == "PRN" || base == "AUX" || base == "NUL";
-            const bool numbered = (base.rfind("COM", 0) == 0 || base.rfind("LPT", 0) == 0) && base.size() == 4 &&
-                                  base[3] >= '1' && base[3] <= '9';
-            if (plain || numbered) st.fail("mesh path " + quoted(text) + " names a device");
-        }
-        std::error_code ec;
-        const auto real_base = std::filesystem::canonical(*base_, ec);
-        if (ec) st.fail("the scene's directory cannot be resolved");
-        const auto real_file = std::filesystem::weakly_canonical(real_base / rel, ec);
-        if (ec) st.fail("cannot resolve mesh path " + quoted(text));
-        auto bi = real_base.begin();
-        auto fi = real_file.begin();
-        for (; bi != real_base.end() && fi != real_file.end() && *bi == *fi; ++bi, ++fi) {
-        }
-        if (bi != real_base.end()) st.fail("mesh path " + quoted(text) + " leads outside the scene's directory");
-        const auto status = std::filesystem::status(real_file, ec);
-        if (status.type() == std::filesystem::file_type::not_found)
-            st.fail("mesh file " + quoted(text) + " does not exist");
-        if (status.type() != std::filesystem::file_type::regular)
-            st.fail("mesh path " + quoted(text) + " is not a regular file");
-        return real_file;
-    }
-
-    void readMesh(Statement& st)
-    {
-        if (!base_) st.fail("meshes can only be read from a scene file, which they sit beside");
-        if (meshes_.size() >= limits_.max_meshes) st.fail("more than " + std::to_string(limits_.max_meshes) + " meshes");
-        const std::string_view name = st.name("a mesh name");
-        if (meshes_.count(std::string(name))) st.fail("mesh " + quoted(name) + " is already defined");
-        const std::string_view path = st.word("a mesh file");
-        const bool smooth = st.peek() == std::optional<std::string_view>("smooth");
-        if (smooth) (void)st.next();
-        st.expectEnd();
-        const auto file = meshFile(st, path);
-        std::ifstream in(file, std::ios::binary);
-        if (!in) st.fail("cannot read mesh file " + quoted(path));
-        std::optional<TriangleMesh> mesh;
-        try {
-            mesh.emplace(readObj(in, mesh_budget_));
-        } catch (const ObjParseError& e) {
-            st.fail("mesh " + quoted(path) + ": " + e.what());
-        } catch (const std::runtime_error&) {
-            st.fail("cannot read mesh file " + quoted(path));
-        } catch (const std::out_of_range& e) {
-            st.fail("mesh " + quoted(path) + ": " + e.what());
-        }
-        std::shared_ptr<const MeshGeometry> geo;
-        try {
-            geo = std::make_shared<const MeshGeometry>(*mesh, smooth ? MeshShading::Smooth : MeshShading::Flat);
-        } catch (const std::invalid_argument&) {
-            st.fail("mesh " + quoted(path) + " has no triangles");
-        }
-        mesh_budget_.max_vertices -= mesh->positions().size();
-        mesh_budget_.max_texture_coordinates -= mesh->textureCoordinates().size();
-        mesh_budget_.max_triangles -= mesh->faces().size();
-        meshes_.emplace(std::string(name), LoadedMesh{std::move(geo)});
-    }
-
-    Transform applied(Statement& st, std::string_view op, const LoadedMesh& mesh, const Transform& placement)
-    {
-        try {
-            if (op == "translate") return Transform::translation(st.vector("the translation")) * placement;
-            if (op == "scale") {
-                const Real first = st.number("a scale factor");
-                const auto next = st.peek();
-                if (!next || isInstanceKeyword(*next)) return Transform::scaling({first, first, first}) * placement;
-                const Real y = st.number("the y scale factor");
-                return Transform::scaling({first, y, st.number("the z scale factor")}) * placement;
-            }
-            if (op == "rotate") {
-                const Vector3r axis = st.vector("the rotation axis");
-                return Transform::rotation(axis, st.number("the rotation angle")) * placement;
-            }
-            if (op == "fit") {
-                const Real size = st.positive("the fitted size");
-                Point3r lo, hi;
-                placement.applyBox(mesh.geometry->boundsMin(), mesh.geometry->boundsMax(), lo, hi);
-                if (!std::isfinite(lo.x) || !std::isfinite(lo.y) || !std::isfinite(lo.z) || !std::isfinite(hi.x) ||
-                    !std::isfinite(hi.y) || !std::isfinite(hi.z))
-                    st.fail("the instance is out of range before 'fit'");
-                const Real ext = std::max({hi.x - lo.x, hi.y - lo.y, hi.z - lo.z});
-                const Real scale = size / ext;
-                if (!std::isfinite(scale) || !(scale > 0)) st.fail("the instance is too large or too small to fit");
-                const Vector3r centre{lo.x / 2 + hi.x / 2, lo.y / 2 + hi.y / 2, lo.z / 2 + hi.z / 2};
-                return Transform::scaling({scale, scale, scale}) *
-                       Transform::translation({-centre.x, -centre.y, -centre.z}) * placement;
-            }
-            if (op == "ground") {
-                const auto& pts = mesh.geometry->usedPoints();
-                if (pts.size() > ground_left_)
-                    st.fail("'ground' would examine more than " + std::to_string(limits_.max_ground_points) +
-                            " points in the scene");
-                ground_left_ -= pts.size();
-                std::optional<Real> lowest;
-                for (const Point3r& p : pts) {
-                    const Real h = placement.apply(p).y;
-                    if (!std::isfinite(h)) st.fail("the instance is out of range before 'ground'");
-                    lowest = lowest ? std::min(*lowest, h) : h;
-                }
-                return Transform::translation({0, -*lowest, 0}) * placement;
-            }
-        } catch (const std::invalid_argument& e) {
-            st.fail(e.what());
-        }
-        st.fail("unknown instance operation " + quoted(op));
-    }
-
-    void readInstance(Statement& st)
-    {
-        countObject(st);
-        const std::string_view mname = st.name("a mesh name");
-        const auto it = meshes_.find(std::string(mname));
-        if (it == meshes_.end()) st.fail("mesh " + quoted(mname) + " is not defined");
-        const Material* m = nullptr;
-        Transform placement;
-        std::size_t ops = 0;
-        while (const auto w = st.next()) {
-            if (*w == "material") {
-                if (m) st.fail("'material' is given twice");
-                m = &materialNamed(st);
-                continue;
-            }
-            if (++ops > limits_.max_instance_operations)
-                st.fail("more than " + std::to_string(limits_.max_instance_operations) + " operations");
-            placement = applied(st, *w, it->second, placement);
-        }
-        if (!m) st.fail("missing 'material'");
-        Point3r lo, hi;
-        placement.applyBox(it->second.geometry->boundsMin(), it->second.geometry->boundsMax(), lo, hi);
-        if (!std::isfinite(lo.x) || !std::isfinite(lo.y) || !std::isfinite(lo.z) || !std::isfinite(hi.x) ||
-            !std::isfinite(hi.y) || !std::isfinite(hi.z))
-            st.fail("mesh instance transform takes the mesh out of range");
-        scene_.add(MeshInstance{it->second.geometry, placement}, *m);
-    }
-
-    SceneLimits limits_;
-    std::optional<std::filesystem::path> base_;
-    ObjLimits mesh_budget_;
-    std::size_t ground_left_;
-    std::map<std::string, LoadedMesh> meshes_;
-    std::map<std::string, Material> materials_;
-    Scene scene_;
-    std::optional<CameraSettings> camera_;
-    bool sky_given_{false};
-    std::size_t objects_{0};
-};
-
+// The reader of reference scene/scene_file.cpp:443-1114, by behaviour: lines of
+// at most max_line_length bytes, '#' starts a comment, one statement per line.
 inline SceneDescription readSceneFrom(std::istream& input, std::optional<std::filesystem::path> base,
                                       const SceneLimits& limits)
 {
     if (base && base->empty()) base = ".";
-    LineReader lines(input, limits.max_line_length);
-    SceneReader reader(limits, std::move(base));
+    SceneState st(limits, std::move(base));
+    LineReader lines(input, limits.max_line_length, "scene");
     std::string line;
     try {
-        while (lines.next(line)) reader.readLine(line, lines.number());
-    } catch (const std::length_error& e) {
-        throw SceneParseError(std::stoull(e.what()),
-                              "line longer than " + std::to_string(limits.max_line_length) + " bytes");
+        while (lines.next(line)) {
+            Cursor c(std::string_view(line).substr(0, line.find('#')), lines.number());
+            const auto keyword = c.next();
+            if (!keyword) continue;
+            const auto kind = std::find_if(kStatements.begin(), kStatements.end(),
+                                           [&](const StatementKind& k) { return k.keyword == *keyword; });
+            if (kind == kStatements.end()) c.fail(text::unknownStatement(*keyword));
+            kind->read(c, st);
+        }
+    } catch (const LineTooLongError& e) {
+        throw SceneParseError(e.line(), "line longer than " + std::to_string(e.limit()) + " bytes");
     }
-    return reader.finish();
+    if (!st.camera) throw SceneParseError(0, text::kNoCamera);
+    return {std::move(st.scene), *st.camera};
 }
 
 } // namespace detail

@@ -300,3 +300,68 @@ def test_million_triangle_scene_reference_sampling_at_4096_spp(ctx, icosphere):
     # 0.065 relative RMSE, 0.026 over 2x2 blocks and 0.4 % in the mean; the bounds
     # are twice those (five times for the mean).
     assert full < 0.13 and coarse < 0.052 and abs(bias) < 0.02
+
+
+def test_full_scene_reference_sampling_at_4096_spp(ctx):
+    # The full-feature scene (checker floor, three smooth trefoil instances under
+    # a two-level BVH, the Perlin marble sphere) at 96x54 and 64x64 samples per
+    # pixel, depth 16, against the oracle in the reference's draw order
+    # (render/render_image.cpp:27-44). The bound is twice the difference between
+    # two independent oracle renders of the same view (seeds 11 and 12), over
+    # pixels and over 2x2 blocks; the mean within 2 %.
+    s = W.full_scene()
+    g = ctx.scene(s)
+    o = oracle.OracleScene(s.to_desc())
+    w, h, n, depth = 96, 54, 64, 16
+    cam = CameraSettings(W.ROOM_CAMERA.look_from, W.ROOM_CAMERA.look_at, (0, 1, 0), 38.0, w / h)
+    img = g.render(cam, w, h, n, max_depth=depth, roulette_start=3, seed=11)
+    ref, _ = o.render(cam.to_abi(), W.render_params(w, h, n, depth, 3, 11), sample_mode=0)
+    ref2, _ = o.render(cam.to_abi(), W.render_params(w, h, n, depth, 3, 12), sample_mode=0)
+
+    def blocks(x):
+        return x.reshape(h // 2, 2, w // 2, 2, 3).mean((1, 3))
+
+    noise, noise_coarse = rel_rmse(ref2, ref), rel_rmse(blocks(ref2), blocks(ref))
+    full, coarse = rel_rmse(img, ref), rel_rmse(blocks(img), blocks(ref))
+    bias = float(img.mean() / ref.mean() - 1)
+    print("full scene at 4096 spp: relative RMSE %.4f (oracle seeds %.4f), 2x2 blocks %.4f (%.4f), mean %+.4f"
+          % (full, noise, coarse, noise_coarse, bias))
+    assert full < 2 * noise and coarse < 2 * noise_coarse and abs(bias) < 0.02
+
+
+def test_batched_job_mix_equals_renders_alone(ctx):
+    # The batched-rendering workload's 64-job mix (seeded room and full-scene
+    # variants with 5..200 random spheres, each job its own scene, depth 8 or 16),
+    # with resolutions and samples shrunk to fit a test: one ab_render_batch call,
+    # every image bit-identical to the job rendered alone, and every room variant
+    # checked against the oracle fed the same random numbers. At 1-9 spp a path
+    # on which FP32 and FP64 take a different bounce (grazing hits on up to 200
+    # small glass and metal spheres) changes its pixel outright, so the check is
+    # on the share of such pixels and on each image's mean: measured 0.3 % of
+    # pixels off by more than 0.1 over the room jobs (1.2 % in the worst job) and
+    # means within 1 %; the bounds are about three and two times that.
+    from albedo_amd import render_batch
+    knot = W.trefoil_mesh()
+    mix = W.job_mix(64, resolutions=((32, 18), (48, 27), (64, 36), (96, 54)), spas=(1, 2, 3), knot=knot)
+    c = W.ROOM_CAMERA
+    jobs = []
+    for scene, (w, h), spa, depth, seed in mix:
+        jobs.append((ctx.scene(scene), CameraSettings(c.look_from, c.look_at, (0, 1, 0), 38.0, w / h), w, h, spa,
+                     depth, 3, seed))
+    outs, st = render_batch(ctx, jobs)
+    assert st["camera_rays"] == sum(j[2] * j[3] * j[4] ** 2 for j in jobs)
+    for j, out in zip(jobs, outs):
+        alone = j[0].render(*j[1:])
+        assert np.array_equal(alone.view(np.uint32), out.view(np.uint32)), j[2:]
+    off, total = 0, 0
+    for k, (scene, (w, h), spa, depth, seed) in enumerate(mix):
+        if depth != 8:
+            continue
+        ref, _ = oracle.OracleScene(scene.to_desc()).render(jobs[k][1].to_abi(),
+                                                            W.render_params(w, h, spa, depth, 3, seed), sample_mode=1)
+        bias = float(outs[k].mean() / ref.mean() - 1)
+        assert abs(bias) < 0.02, (k, bias)
+        off += int((np.abs(outs[k] - ref).max(-1) > 0.1).sum())
+        total += w * h
+    print("room variants: %d of %d pixels off by more than 0.1" % (off, total))
+    assert off < 0.01 * total
