@@ -41,11 +41,19 @@ uint32_t gridFor(uint32_t configured, uint32_t capacity, uint32_t block)
 // Traversal stack class of the bounce kernels for a scene stack depth.
 uint32_t stackClass(uint32_t depth) { return depth <= kShortStack ? kShortStack : (depth <= 32 ? 32u : 64u); }
 
+// Block size and grid for the shape's one-ray-per-lane stack class.
+void settleBlocks(LaunchShape& s, const ab_launch_config& cfg, uint32_t capacity)
+{
+    s.block = cfg.block_size;
+    if (s.block == 256 && s.pstack > 32) s.block = 128; // 256 x 64 stack entries exceed the LDS budget
+    s.grid = gridFor(cfg.grid_blocks, capacity, s.block);
+}
+
 LaunchShape shapeFor(const ab_launch_config& cfg, uint32_t capacity, uint32_t stack)
 {
     LaunchShape s{cfg.block_size, cfg.grid_blocks, stackClass(stack)};
-    if (s.block == 256 && s.stack > 32) s.block = 128; // 256 x 64 stack entries exceed the LDS budget
-    s.grid = gridFor(s.grid, capacity, s.block);
+    s.pstack = s.stack;
+    settleBlocks(s, cfg, capacity);
     return s;
 }
 
@@ -56,12 +64,12 @@ constexpr uint32_t kQueueRaysPerLane = 6;
 // Extension and shadow rays are traced with lane refill where incoherent rays
 // through triangle BLASes dominate: over the one scene of every job when that
 // scene has mesh instances (`one_scene`: the set holds every scene whose paths
-// may be in flight); in a queue of several scenes, the rays of mesh-only
-// scenes (split), each lane over its job's scene. Rays of scenes with spheres
-// keep the one-ray-per-lane kernels, which were faster on them (shallow trees
-// of spheres: refill cost the sphere room 50 %, and the serving mix 11 % when
-// it took every ray).
-void chooseTracing(LaunchShape& s, const SceneSet& set, uint32_t capacity, bool one_scene)
+// may be in flight); in a queue of several scenes, the rays of the scenes with
+// meshes (split), a wave holding one scene at a time (kTraceGrouped). Rays of
+// sphere-only scenes keep the one-ray-per-lane kernels, which were faster on
+// them (shallow trees of spheres: refill cost the sphere room 50 %).
+void chooseTracing(LaunchShape& s, const SceneSet& set, uint32_t capacity, bool one_scene,
+                   const ab_launch_config& cfg)
 {
     const uint32_t qgrid =
         std::max<uint32_t>((capacity + 64 * kQueueRaysPerLane - 1) / (64 * kQueueRaysPerLane), 1);
@@ -71,17 +79,19 @@ void chooseTracing(LaunchShape& s, const SceneSet& set, uint32_t capacity, bool 
         s.q_spheres = set.first.has_spheres ? 1u : 0u;
         s.q_reg = set.first.stack_depth + 1 > s.stack ? 1u : 0u;
         s.qgrid = qgrid;
-    } else if (set.mesh_only) {
+        s.pstack = kShortStack; // no ray goes one per lane
+        settleBlocks(s, cfg, capacity);
+    } else if (set.meshes) {
         s.refill = kTraceSplit;
-        s.q_spheres = 0;
-        s.q_reg = set.deepest_mesh_only + 1 > s.stack ? 1u : 0u;
+        s.q_spheres = set.mesh_spheres ? 1u : 0u;
+        s.q_reg = set.deepest_meshes + 1 > s.stack ? 1u : 0u;
         s.qgrid = qgrid;
     }
 }
 
 bool sameTracing(const LaunchShape& a, const LaunchShape& b)
 {
-    if (a.refill != b.refill) return false;
+    if (a.refill != b.refill || a.pstack != b.pstack || a.block != b.block) return false;
     if (a.refill == kTracePlain) return true;
     return a.qgrid == b.qgrid && a.q_spheres == b.q_spheres && a.q_reg == b.q_reg &&
            std::memcmp(&a.scene, &b.scene, sizeof(DevScene)) == 0;
@@ -166,6 +176,27 @@ ab_status prepareJob(const JobSpec& spec, DevJob& dj)
 
 Engine::~Engine() { freeAll(); }
 
+void Engine::clearTags() { tags_.clear(); }
+
+ab_status Engine::tagFor(const DevScene& scene, uint32_t& tag)
+{
+    tag = 0;
+    if (!scene.has_meshes) return AB_OK;
+    for (size_t t = 0; t < tags_.size(); ++t)
+        if (std::memcmp(&tags_[t], &scene, sizeof(DevScene)) == 0) {
+            tag = uint32_t(t + 1);
+            return AB_OK;
+        }
+    if (tags_.size() + 1 >= kMaxTags) return AB_OK; // no tag left: traced one ray per lane
+    tags_.push_back(scene);
+    tag = uint32_t(tags_.size());
+    // The host copy stays in tags_ (never moved: reserved to kMaxTags entries)
+    // until the stream has run this copy.
+    HIP_TRY(hipMemcpyAsync(d_tags_ + tag, &tags_.back(), sizeof(DevScene), hipMemcpyHostToDevice, stream_),
+            "hipMemcpy(scene tag)");
+    return AB_OK;
+}
+
 void Engine::freeAll()
 {
     if (graph_) (void)hipGraphExecDestroy(graph_);
@@ -219,6 +250,7 @@ ab_status Engine::reserve(uint32_t capacity, uint32_t max_jobs, hipStream_t)
                     {reinterpret_cast<void**>(&p.block_off), size_t(kMaxGrid) * 8 * 4},
                     {reinterpret_cast<void**>(&p.ctr), sizeof(Counters)},
                     {reinterpret_cast<void**>(&d_jobs_), sizeof(DevJob) * std::max<uint32_t>(max_jobs, 1)},
+                    {reinterpret_cast<void**>(&d_tags_), sizeof(DevScene) * kMaxTags},
                     {reinterpret_cast<void**>(&d_segs_), sizeof(DevSegment) * seg_cap}};
     size_t total = 0;
     for (const Part& part : parts) total += align256(part.bytes);
@@ -230,6 +262,9 @@ ab_status Engine::reserve(uint32_t capacity, uint32_t max_jobs, hipStream_t)
     }
     p.capacity = capacity;
     p.jobs = d_jobs_;
+    p.tag_scenes = d_tags_;
+    tags_.clear();
+    tags_.reserve(kMaxTags);
     p.segs = d_segs_;
     p.seg_mask = seg_cap - 1;
     seg_cap_ = seg_cap;
@@ -269,7 +304,7 @@ ab_status Engine::capture(const LaunchShape& shape)
     // The graph bakes in the path-state pointers, which change only when
     // reserve() reallocates (and then drops the graph).
     if (graph_ && shape.block == graph_shape_.block && shape.grid == graph_shape_.grid &&
-        shape.stack == graph_shape_.stack && sameTracing(shape, graph_shape_))
+        shape.stack == graph_shape_.stack && shape.pstack == graph_shape_.pstack && sameTracing(shape, graph_shape_))
         return AB_OK;
     if (graph_) {
         (void)hipGraphExecDestroy(graph_);
@@ -299,7 +334,7 @@ LaunchShape singleSceneShape(const ab_launch_config& cfg, uint32_t capacity, con
     LaunchShape shape = shapeFor(cfg, capacity, scene.stack_depth);
     SceneSet set;
     set.add(scene);
-    chooseTracing(shape, set, capacity, true);
+    chooseTracing(shape, set, capacity, true, cfg);
     return shape;
 }
 
@@ -403,6 +438,23 @@ ab_status Engine::renderBatch(const std::vector<JobSpec>& specs, const std::vect
         off += bufs[i].total;
     }
     work_total_ = total;
+    LaunchShape shape = shapeFor(cfg, capacity, stack);
+    SceneSet set;
+    for (const DevJob& j : jobs_) set.add(j.scene);
+    chooseTracing(shape, set, capacity, true, cfg);
+    // Split tracing: the jobs of each scene with meshes share a scene tag, and
+    // the one-ray-per-lane kernels take the stack class of the untagged scenes.
+    clearTags();
+    if (shape.refill == kTraceSplit) {
+        uint32_t plain_depth = 0;
+        for (DevJob& j : jobs_) {
+            st = tagFor(j.scene, j.tag);
+            if (st != AB_OK) return st;
+            if (j.tag == 0) plain_depth = std::max(plain_depth, j.scene.stack_depth);
+        }
+        shape.pstack = stackClass(plain_depth);
+        settleBlocks(shape, cfg, capacity);
+    }
     HIP_TRY(hipMemcpyAsync(d_jobs_, jobs_.data(), sizeof(DevJob) * jobs_.size(), hipMemcpyHostToDevice, stream_),
             "hipMemcpy(jobs)");
     // One segment per job, in job order (the ring holds at least max_jobs entries).
@@ -416,10 +468,6 @@ ab_status Engine::renderBatch(const std::vector<JobSpec>& specs, const std::vect
     seg_hi_ = seg_sent_ = uint32_t(jobs_.size());
     HIP_TRY(hipMemcpyAsync(d_segs_, h_segs_, sizeof(DevSegment) * jobs_.size(), hipMemcpyHostToDevice, stream_),
             "hipMemcpy(segments)");
-    LaunchShape shape = shapeFor(cfg, capacity, stack);
-    SceneSet set;
-    for (const DevJob& j : jobs_) set.add(j.scene);
-    chooseTracing(shape, set, capacity, true);
     // The timing events, destroyed on every way out of this function.
     struct Events {
         hipEvent_t e[2]{nullptr, nullptr};
@@ -485,6 +533,7 @@ ab_status Engine::beginStream(const ab_launch_config& cfg, uint32_t max_jobs)
     if (st != AB_OK) return st;
     jobs_.clear();
     jobs_.reserve(max_jobs_);
+    clearTags();
     work_total_ = 0;
     seg_lo_ = seg_hi_ = seg_sent_ = 0;
     stream_cfg_ = cfg;
@@ -496,7 +545,7 @@ ab_status Engine::beginStream(const ab_launch_config& cfg, uint32_t max_jobs)
 ab_status Engine::traceScenes(const SceneSet* live)
 {
     LaunchShape next = shapeFor(stream_cfg_, capacity_, shape_.stack);
-    chooseTracing(next, live ? *live : table_set_, capacity_, live != nullptr);
+    chooseTracing(next, live ? *live : table_set_, capacity_, live != nullptr, stream_cfg_);
     if (sameTracing(next, shape_)) return AB_OK;
     shape_ = next;
     return capture(shape_);
@@ -505,7 +554,7 @@ ab_status Engine::traceScenes(const SceneSet* live)
 ab_status Engine::setStreamStack(uint32_t stack)
 {
     shape_ = shapeFor(stream_cfg_, capacity_, stack);
-    chooseTracing(shape_, table_set_, capacity_, false);
+    chooseTracing(shape_, table_set_, capacity_, false, stream_cfg_);
     return capture(shape_);
 }
 
@@ -522,7 +571,7 @@ ab_status Engine::appendJob(const DevJob& prepared, unsigned long long* accum, u
     table_set_.add(prepared.scene);
     const uint32_t stack = std::max(shape_.stack, stackClass(prepared.scene.stack_depth));
     LaunchShape next = shapeFor(stream_cfg_, capacity_, stack);
-    chooseTracing(next, table_set_, capacity_, false);
+    chooseTracing(next, table_set_, capacity_, false, stream_cfg_);
     if (next.stack != shape_.stack || !sameTracing(next, shape_)) {
         const ab_status st = setStreamStack(stack);
         if (st != AB_OK) {
@@ -533,6 +582,11 @@ ab_status Engine::appendJob(const DevJob& prepared, unsigned long long* accum, u
     DevJob dj = prepared;
     dj.accum = accum;
     dj.done = done;
+    // Every job of a scene with meshes gets that scene's tag while tags last, so
+    // whichever way the bounces trace (they switch with the scenes in flight), its
+    // paths' queue entries name their scene.
+    const ab_status tst = tagFor(dj.scene, dj.tag);
+    if (tst != AB_OK) return tst;
     const uint32_t idx = uint32_t(jobs_.size());
     jobs_.push_back(dj);
     HIP_TRY(hipMemcpyAsync(d_jobs_ + idx, &jobs_[idx], sizeof(DevJob), hipMemcpyHostToDevice, stream_),
@@ -596,6 +650,7 @@ ab_status Engine::runBounces(uint32_t count)
 ab_status Engine::resetTable()
 {
     jobs_.clear();
+    clearTags();
     work_total_ = 0;
     seg_lo_ = seg_hi_ = seg_sent_ = 0;
     HIP_TRY(launchReset(ps_, 0, 0, stream_), "reset launch");
@@ -674,22 +729,30 @@ ab_status ab_autotune(ab_context* ctx, const ab_job_desc* jobs, uint32_t n_jobs,
     if (!ctx || !jobs || n_jobs == 0) return fail(AB_INVALID_ARGUMENT, "ab_autotune: no jobs");
     if (mode != AB_TUNE_PLAIN && mode != AB_TUNE_BATCH)
         return fail(AB_INVALID_ARGUMENT, "ab_autotune: mode must be AB_TUNE_PLAIN or AB_TUNE_BATCH");
-    uint64_t pixels = 0;
+    uint64_t samples = 0;
     for (uint32_t i = 0; i < n_jobs; ++i) {
         if (!jobs[i].scene || jobs[i].scene->ctx != ctx)
             return fail(AB_INVALID_ARGUMENT, "ab_autotune: job without a scene of this context");
-        pixels += uint64_t(jobs[i].params.width) * jobs[i].params.height;
+        const uint64_t spa = jobs[i].params.samples_per_axis;
+        samples += uint64_t(jobs[i].params.width) * jobs[i].params.height * spa * spa;
     }
-    // Trial size: the jobs' own resolutions, samples per axis capped so the
-    // whole trial is at most 2^24 camera samples.
-    uint32_t cap = 1;
-    while (uint64_t(cap + 1) * (cap + 1) * pixels <= (uint64_t(1) << 24)) ++cap;
+    // Trial size: every job keeps its scene, camera (aspect ratio included) and
+    // samples per pixel and renders a band of its rows, the same share of each
+    // job's rows, so one trial takes about kTuneSamples camera samples. The queue
+    // then holds jobs of the same sizes relative to each other, and pixels of the
+    // same sample counts, as the real run; cutting samples per pixel instead would
+    // leave every job so small that the whole list sits in the queue at once.
+    // Trials of 2^26 samples on the 64-job batched workload (2.4 G samples)
+    // ranked 4M paths in flight first, against 2M in the full run; from 2^28 the
+    // ranking matched.
+    constexpr uint64_t kTuneSamples = uint64_t(1) << 28;
+    const double share = samples > kTuneSamples ? double(kTuneSamples) / double(samples) : 1.0;
     std::vector<JobSpec> specs;
     std::vector<std::vector<float>> images(n_jobs);
     std::vector<float*> outs;
     for (uint32_t i = 0; i < n_jobs; ++i) {
         JobSpec s{jobs[i].scene, jobs[i].camera, jobs[i].params};
-        s.params.samples_per_axis = std::min(std::max<uint32_t>(s.params.samples_per_axis, 1), cap);
+        s.params.height = std::max<uint32_t>(1, uint32_t(std::ceil(double(s.params.height) * share)));
         DevJob check{};
         const ab_status st = prepareJob(s, check);
         if (st != AB_OK) return st;
@@ -727,9 +790,14 @@ ab_status ab_autotune(ab_context* ctx, const ab_job_desc* jobs, uint32_t n_jobs,
         for (uint32_t p : paths)
             for (uint32_t g : grids) {
                 const ab_launch_config cfg{b, p, g, base.use_graph};
-                // The first run of a shape allocates and captures; time the second.
+                // The first render under a configuration allocates its path state
+                // and captures its bounce graph: a one-row render of the first job
+                // does that before the timed trial.
+                JobSpec warm = specs[0];
+                warm.params.height = 1;
+                float* row = images[0].data();
+                ab_status st = engine.renderBatch({warm}, {row}, nullptr, cfg);
                 double ms = 0;
-                ab_status st = trial(cfg, ms);
                 if (st == AB_OK) st = trial(cfg, ms);
                 if (st != AB_OK) return st;
                 if (ms < winner_ms) {

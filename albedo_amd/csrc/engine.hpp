@@ -42,18 +42,19 @@ inline JobBuffers jobBuffers(const DevJob& j)
 
 // What the bounce kernels must handle of a set of jobs' scenes.
 struct SceneSet {
-    bool meshes{false}, mesh_only{false}, same{true}, empty{true};
-    uint32_t deepest_mesh_only{0}; // largest stack bound of the mesh-only scenes
+    bool meshes{false}, same{true}, empty{true};
+    bool mesh_spheres{false};  // a scene with meshes also has spheres
+    uint32_t deepest_meshes{0}; // largest stack bound of the scenes with meshes
     DevScene first{};
     void add(const DevScene& s)
     {
         if (empty) first = s;
         else if (std::memcmp(&s, &first, sizeof(DevScene)) != 0) same = false;
         empty = false;
-        meshes = meshes || s.has_meshes;
-        if (s.has_meshes && !s.has_spheres) {
-            mesh_only = true;
-            deepest_mesh_only = deepest_mesh_only > s.stack_depth ? deepest_mesh_only : s.stack_depth;
+        if (s.has_meshes) {
+            meshes = true;
+            mesh_spheres = mesh_spheres || s.has_spheres;
+            deepest_meshes = deepest_meshes > s.stack_depth ? deepest_meshes : s.stack_depth;
         }
     }
 };
@@ -111,6 +112,10 @@ private:
     ab_status capture(const LaunchShape& shape);
     // Serving: switches the bounce kernels to the stack class of `stack`.
     ab_status setStreamStack(uint32_t stack);
+    // The scene tag of `scene` (kSlotBits): a new one when the scene has meshes
+    // and a tag is free, written to the device table on the stream; 0 otherwise.
+    ab_status tagFor(const DevScene& scene, uint32_t& tag);
+    void clearTags();
     void freeAll();
 
     ab_context* ctx_;
@@ -129,6 +134,8 @@ private:
     uint32_t seg_lo_{0}, seg_hi_{0}; // live segments, logical indices
     uint32_t seg_sent_{0};          // segments below this index are on the device
     std::vector<DevJob> jobs_;
+    DevScene* d_tags_{nullptr};        // kMaxTags scenes on the device (PathState::tag_scenes)
+    std::vector<DevScene> tags_;       // the tags given out, tag t at index t - 1
     unsigned long long work_total_{0};
     LaunchShape shape_{};          // serving: current bounce shape
     SceneSet table_set_{};         // serving: the scenes of the job table

@@ -115,7 +115,7 @@ __global__ void kGenerate(PathState ps)
         // No hit until the path is traced: a slot's previous path's hit must
         // never reach shading (it may index another scene's arrays).
         ps.hit[slot] = make_uint4(0x7f800000u, kMissId, kMissId, 0u);
-        ps.active[act0 + i] = slot;
+        ps.active[act0 + i] = slot | (job.tag << kSlotBits);
     }
 }
 
@@ -265,11 +265,8 @@ __device__ __forceinline__ uint32_t materialBin(const DevScene& sc, uint32_t obj
     return sc.materials[sc.objects[obj].material].kind + 1;
 }
 
-// Scenes whose rays kTraceQueue takes in a queue of several scenes (SPLIT).
-__device__ __forceinline__ bool meshOnly(const DevScene& sc) { return sc.has_meshes && !sc.has_spheres; }
-
-// SPLIT: rays of mesh-only scenes are left to kTraceQueue (their keys too; the
-// block counts then come from kCountBins).
+// SPLIT: rays whose queue entries carry a scene tag are left to kTraceGrouped
+// (their keys too; the block counts then come from kCountBins).
 template <int BLOCK, int STACK, bool SPLIT>
 __global__ __launch_bounds__(BLOCK) void kExtend(PathState ps)
 {
@@ -280,9 +277,10 @@ __global__ __launch_bounds__(BLOCK) void kExtend(PathState ps)
     for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
         const uint32_t i = base + threadIdx.x;
         uint32_t flags = 0;
-        const uint32_t slot = i < n ? ps.active[i] : 0u;
-        const uint32_t j = i < n ? __float_as_uint(ps.prev[slot].w) : 0u;
-        if (i < n && !(SPLIT && meshOnly(ps.jobs[j].scene))) {
+        const uint32_t entry = i < n ? ps.active[i] : 0u;
+        const uint32_t slot = entry & kSlotMask;
+        if (i < n && !(SPLIT && (entry >> kSlotBits))) {
+            const uint32_t j = __float_as_uint(ps.prev[slot].w);
             const float4 o = ps.org[slot], d = ps.dir[slot];
             const DevScene& sc = ps.jobs[j].scene;
             const RayF r{o.x, o.y, o.z, 0.0f, d.x, d.y, d.z, __builtin_huge_valf()};
@@ -664,7 +662,7 @@ __global__ __launch_bounds__(BLOCK) void kShade(PathState ps)
     const uint32_t n = ps.ctr->n_in;
     for (uint32_t k = gridStart(); k < n; k += gridStride()) {
         const uint32_t i = ps.binned[k];
-        ps.key[i] = shadePath(ps, ps.active[i]);
+        ps.key[i] = shadePath(ps, ps.active[i] & kSlotMask);
     }
 }
 
@@ -738,10 +736,10 @@ __global__ __launch_bounds__(BLOCK) void kScatterShade(PathState ps)
         uint32_t rank[3], total[3];
         blockRanks<BLOCK, 3>(flags, rank, total);
         if (i < n) {
-            const uint32_t slot = ps.active[i];
-            if (flags & kQueueActive) ps.active_next[run[0] + rank[0]] = slot;
-            if (flags & kQueueShadow) ps.shadowq[run[1] + rank[1]] = slot;
-            if (flags & kQueueTerm) ps.termq[run[2] + rank[2]] = slot;
+            const uint32_t entry = ps.active[i]; // the slot and its scene tag
+            if (flags & kQueueActive) ps.active_next[run[0] + rank[0]] = entry;
+            if (flags & kQueueShadow) ps.shadowq[run[1] + rank[1]] = entry;
+            if (flags & kQueueTerm) ps.termq[run[2] + rank[2]] = entry & kSlotMask;
         }
 #pragma unroll
         for (int q = 0; q < 3; ++q) run[q] += total[q];
@@ -752,16 +750,17 @@ __global__ __launch_bounds__(BLOCK) void kScatterShade(PathState ps)
 
 // Occlusion of the deferred light samples (reference Scene::occluded,
 // scene/scene.cpp:157-165); unoccluded contributions join the path's radiance.
-// SPLIT: rays of mesh-only scenes are left to kTraceQueue.
+// SPLIT: rays whose queue entries carry a scene tag are left to kTraceGrouped.
 template <int BLOCK, int STACK, bool SPLIT>
 __global__ __launch_bounds__(BLOCK) void kShadow(PathState ps)
 {
     __shared__ int32_t stack[STACK * BLOCK];
     const uint32_t n = ps.ctr->n_shadow;
     for (uint32_t i = gridStart(); i < n; i += gridStride()) {
-        const uint32_t slot = ps.shadowq[i];
+        const uint32_t entry = ps.shadowq[i];
+        if (SPLIT && (entry >> kSlotBits)) continue;
+        const uint32_t slot = entry & kSlotMask;
         const uint32_t j = __float_as_uint(ps.prev[slot].w);
-        if (SPLIT && meshOnly(ps.jobs[j].scene)) continue;
         const float4 o = ps.sh_org[slot], d = ps.sh_dir[slot];
         const RayF r{o.x, o.y, o.z, 0.0f, d.x, d.y, d.z, o.w};
         HitF h;
@@ -778,27 +777,50 @@ __global__ __launch_bounds__(BLOCK) void kShadow(PathState ps)
 
 // ------------------------------------------------- extend and shadow, lane refill
 
-// For queues with mesh-instance scenes: the extension and shadow rays traced as
-// the ray-query kernel traces them, over one scene passed as a kernel argument
-// or (MIXED, a queue of several scenes) the rays of mesh-only scenes, each lane
-// over its job's scene; kExtend and kShadow take the others. Each one-wave block owns a contiguous slice of the queue, and a
-// lane whose ray has finished takes the next ray of the slice once kQueueRefill
-// lanes are free, so a wave no longer waits for its longest ray. RayQuery
-// visits the same nodes and primitives as traceRay, so the hits and shadow
-// results are those of kExtend and kShadow. The material keys are counted
-// afterwards by kCountBins, over the block partition kScatterBins uses.
+// For queues whose rays traverse mesh instances: the extension and shadow rays
+// traced as the ray-query kernel traces them. Each one-wave block owns a
+// contiguous slice of the queue, and a lane whose ray has finished takes the
+// next ray of the slice once kQueueRefill lanes are free, so a wave no longer
+// waits for its longest ray. RayQuery visits the same nodes and primitives as
+// traceRay, so the hits and shadow results are those of kExtend and kShadow. The
+// material keys are counted afterwards by kCountBins, over the block partition
+// kScatterBins uses.
 constexpr uint32_t kQueueRefill = 16; // free lanes that trigger a refill
 constexpr uint32_t kQueueSteps = 16;  // traversal steps between refill checks
 
-template <bool MIXED>
-__device__ __forceinline__ const DevScene& laneScene(const PathState& ps, uint32_t job, const DevScene& one)
+// Writes the finished query of the ray in `slot` at queue position `pos`.
+template <bool ANY, class Query>
+__device__ __forceinline__ void finishQuery(const PathState& ps, const DevScene& sc, const Query& q, uint32_t slot,
+                                            uint32_t pos)
 {
-    if constexpr (MIXED) return ps.jobs[job].scene;
-    else return one;
+    if (ANY) {
+        if (q.best.object == kMissId) {
+            const float4 cc = ps.sh_c[slot];
+            float4 rr = ps.rad[slot];
+            rr.x += cc.x;
+            rr.y += cc.y;
+            rr.z += cc.z;
+            ps.rad[slot] = rr;
+        }
+    } else {
+        const HitF& h = q.best;
+        ps.hit[slot] = make_uint4(__float_as_uint(h.t), h.object, h.prim, __float_as_uint(h.b1));
+        ps.hit_b2[slot] = h.b2;
+        ps.key[pos] = 1u << materialBin(sc, h.object);
+    }
 }
 
-template <int STACK, bool ANY, bool SPHERES, bool LEAF_IN_REG, bool MIXED>
-__global__ __launch_bounds__(64) void kTraceQueue(PathState ps, DevScene one)
+template <bool ANY>
+__device__ __forceinline__ RayF queuedRay(const PathState& ps, uint32_t slot)
+{
+    const float4 o = ANY ? ps.sh_org[slot] : ps.org[slot];
+    const float4 d = ANY ? ps.sh_dir[slot] : ps.dir[slot];
+    return RayF{o.x, o.y, o.z, 0.0f, d.x, d.y, d.z, ANY ? o.w : __builtin_huge_valf()};
+}
+
+// Every ray of the queue over one scene, passed as a kernel argument.
+template <int STACK, bool ANY, bool SPHERES, bool LEAF_IN_REG>
+__global__ __launch_bounds__(64) void kTraceQueue(PathState ps, DevScene sc)
 {
     __shared__ int32_t stack[STACK * 64];
     const Counters* c = ps.ctr;
@@ -811,7 +833,7 @@ __global__ __launch_bounds__(64) void kTraceQueue(PathState ps, DevScene one)
     const LdsStack<64> stk{stack + threadIdx.x};
     RayQuery<64, ANY, SPHERES, LEAF_IN_REG> q;
     constexpr uint32_t kNone = ~0u;
-    uint32_t id = kNone, slot = 0, job = 0;
+    uint32_t id = kNone, slot = 0;
     while (true) {
         const bool idle = id == kNone;
         const unsigned long long free_lanes = __ballot(idle);
@@ -819,41 +841,128 @@ __global__ __launch_bounds__(64) void kTraceQueue(PathState ps, DevScene one)
             const uint32_t mine = next + uint32_t(__popcll(free_lanes & ((1ull << lane) - 1ull)));
             next += uint32_t(__popcll(free_lanes));
             if (idle && mine < end) {
-                slot = ANY ? ps.shadowq[mine] : ps.active[mine];
-                if (MIXED) job = __float_as_uint(ps.prev[slot].w);
-                if (!MIXED || meshOnly(ps.jobs[job].scene)) {
-                    const float4 o = ANY ? ps.sh_org[slot] : ps.org[slot];
-                    const float4 d = ANY ? ps.sh_dir[slot] : ps.dir[slot];
-                    q.begin(laneScene<MIXED>(ps, job, one),
-                            RayF{o.x, o.y, o.z, 0.0f, d.x, d.y, d.z, ANY ? o.w : __builtin_huge_valf()});
-                    id = mine;
-                }
+                slot = (ANY ? ps.shadowq[mine] : ps.active[mine]) & kSlotMask;
+                q.begin(sc, queuedRay<ANY>(ps, slot));
+                id = mine;
             }
         }
-        // Done when no lane holds a ray and the slice is used up. (MIXED: a refill
-        // can leave every lane idle, all its rays being the other kernels', while
-        // the slice still holds rays; `next` is the same in all lanes and grows by
-        // 64 at each such pass, so the loop ends.)
+        // Done when no lane holds a ray and the slice is used up.
         if (__ballot(id != kNone) == 0 && next >= end) break;
         if (id != kNone) {
-            const DevScene& sc = laneScene<MIXED>(ps, job, one);
             for (uint32_t k = 0; k < kQueueSteps && !q.done; ++k) q.step(sc, stk);
             if (q.done) {
-                if (ANY) {
-                    if (q.best.object == kMissId) {
-                        const float4 cc = ps.sh_c[slot];
-                        float4 rr = ps.rad[slot];
-                        rr.x += cc.x;
-                        rr.y += cc.y;
-                        rr.z += cc.z;
-                        ps.rad[slot] = rr;
-                    }
+                finishQuery<ANY>(ps, sc, q, slot, id);
+                id = kNone;
+            }
+        }
+    }
+}
+
+// A value the whole wave holds the same, moved to a scalar register.
+__device__ __forceinline__ uint32_t waveUniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+template <class T>
+__device__ __forceinline__ T* waveUniform(T* p)
+{
+    const auto bits = reinterpret_cast<unsigned long long>(p);
+    const unsigned long long lo = waveUniform(uint32_t(bits)), hi = waveUniform(uint32_t(bits >> 32));
+    return reinterpret_cast<T*>((hi << 32) | lo);
+}
+
+// The scene of tag `tag`, read once into scalar registers: traversal then finds
+// its arrays there, as it finds a kernel argument's, instead of loading them
+// from memory at every step. `tag` must be the same in every lane.
+__device__ __forceinline__ DevScene sceneOfTag(const PathState& ps, uint32_t tag)
+{
+    const DevScene& s = ps.tag_scenes[tag];
+    DevScene u;
+    u.nodes = waveUniform(s.nodes);
+    u.tris = waveUniform(s.tris);
+    u.leafobj = waveUniform(s.leafobj);
+    u.objects = waveUniform(s.objects);
+    u.spheres = waveUniform(s.spheres);
+    u.shading = waveUniform(s.shading);
+    u.materials = waveUniform(s.materials);
+    u.lights = waveUniform(s.lights);
+    u.tlas_root = int32_t(waveUniform(uint32_t(s.tlas_root)));
+    u.num_objects = waveUniform(s.num_objects);
+    u.num_lights = waveUniform(s.num_lights);
+    u.sky_light = int32_t(waveUniform(uint32_t(s.sky_light)));
+    u.stack_depth = waveUniform(s.stack_depth);
+    u.has_spheres = waveUniform(s.has_spheres);
+    u.has_meshes = waveUniform(s.has_meshes);
+    return u;
+}
+
+// In a queue of several scenes, the rays whose queue entries carry a scene tag
+// (the others go to kExtend / kShadow), one scene per wave at a time. Queues
+// keep the order in which samples were generated, so a job's rays form runs.
+// When every lane is idle the wave starts a run at its slice's next tagged ray,
+// skipping the other kernels' rays, and holds that run's scene in wave-uniform
+// registers; later refills take only the following rays of the same tag and
+// stop at the first ray of another. Traversal then reads one scene per wave from
+// scalar registers, as kTraceQueue does with its kernel argument, instead of a
+// scene per lane; the ownership and run checks read only the queue entries.
+template <int STACK, bool ANY, bool SPHERES, bool LEAF_IN_REG>
+__global__ __launch_bounds__(64) void kTraceGrouped(PathState ps)
+{
+    __shared__ int32_t stack[STACK * 64];
+    const Counters* c = ps.ctr;
+    const uint32_t n = ANY ? c->n_shadow : c->n_active + pendingGeneration(c).n_new;
+    const uint32_t per_lane = (n + 64 * gridDim.x - 1) / (64 * gridDim.x);
+    const uint32_t slice = per_lane * 64;
+    uint32_t next = xcdRemap(blockIdx.x, gridDim.x) * slice;
+    const uint32_t end = next + slice < n ? next + slice : n;
+    const uint32_t lane = __lane_id();
+    const LdsStack<64> stk{stack + threadIdx.x};
+    RayQuery<64, ANY, SPHERES, LEAF_IN_REG> q;
+    constexpr uint32_t kNone = ~0u;
+    uint32_t id = kNone, slot = 0;
+    DevScene sc{};        // the run's scene (wave-uniform)
+    uint32_t run_tag = 0; // ... and its tag
+    while (true) {
+        const bool idle = id == kNone;
+        const unsigned long long free_lanes = __ballot(idle);
+        const uint32_t n_free = uint32_t(__popcll(free_lanes));
+        if (next < end && (n_free >= kQueueRefill || free_lanes == ~0ull)) {
+            // Lane i looks at window position next + i: its entry's scene tag
+            // (0: another kernel's ray).
+            const uint32_t cnt = min(n_free, end - next);
+            const uint32_t w_entry = lane < cnt ? (ANY ? ps.shadowq[next + lane] : ps.active[next + lane]) : 0u;
+            const uint32_t w_tag = w_entry >> kSlotBits;
+            uint32_t start = 0;
+            bool fill = true;
+            if (free_lanes == ~0ull) {
+                const unsigned long long owned = __ballot(w_tag != 0u);
+                if (owned == 0ull) {
+                    next += cnt; // none of ours in the window
+                    fill = false;
                 } else {
-                    const HitF& h = q.best;
-                    ps.hit[slot] = make_uint4(__float_as_uint(h.t), h.object, h.prim, __float_as_uint(h.b1));
-                    ps.hit_b2[slot] = h.b2;
-                    ps.key[id] = 1u << materialBin(sc, h.object);
+                    start = uint32_t(__ffsll(static_cast<long long>(owned))) - 1;
+                    run_tag = waveUniform(uint32_t(__shfl(w_tag, int(start))));
+                    sc = sceneOfTag(ps, run_tag);
                 }
+            }
+            if (fill) {
+                // The run goes on up to the first window position of another scene.
+                const unsigned long long other = __ballot(lane < cnt && lane >= start && w_tag != run_tag);
+                const uint32_t take = other ? uint32_t(__ffsll(static_cast<long long>(other))) - 1 : cnt;
+                const uint32_t k = start + uint32_t(__popcll(free_lanes & ((1ull << lane) - 1ull)));
+                if (idle && k < take) {
+                    slot = (ANY ? ps.shadowq[next + k] : ps.active[next + k]) & kSlotMask;
+                    q.begin(sc, queuedRay<ANY>(ps, slot));
+                    id = next + k;
+                }
+                next += take;
+            }
+        }
+        // Done when no lane holds a ray and the slice is used up: each pass with
+        // every lane idle either starts a run (taking at least one ray) or moves
+        // past a window of other kernels' rays, so `next` reaches `end`.
+        if (__ballot(id != kNone) == 0 && next >= end) break;
+        if (id != kNone) {
+            for (uint32_t k = 0; k < kQueueSteps && !q.done; ++k) q.step(sc, stk);
+            if (q.done) {
+                finishQuery<ANY>(ps, sc, q, slot, id);
                 id = kNone;
             }
         }
@@ -993,28 +1102,38 @@ __global__ void kAddWork(PathState ps, unsigned long long total, uint32_t seg_lo
     c->seg_hi = seg_hi;
 }
 
-// The lane-refill tracing kernel for the shape: one scene or each lane's job's
-// scene (MIXED), with the FP64 sphere test only if a scene has spheres, and the
-// rest of a TLAS leaf in a register when that extra stack entry would not fit
-// the stack class.
-template <int STACK, bool ANY, bool MIXED>
-void launchTraceQueueM(const PathState& ps, const LaunchShape& sh, hipStream_t s)
+// The lane-refill tracing kernel for the shape: over its one scene, or grouped
+// by scene (split tracing), with the FP64 sphere test only if a scene it traces
+// has spheres, and the rest of a TLAS leaf in a register when that extra stack
+// entry would not fit the stack class.
+template <int STACK, bool ANY, bool SPHERES, bool REG>
+void launchRefill(const PathState& ps, const LaunchShape& sh, hipStream_t s)
 {
     const dim3 g(sh.qgrid), b(64);
-    if (sh.q_spheres) {
-        if (sh.q_reg) hipLaunchKernelGGL((kTraceQueue<STACK, ANY, true, true, MIXED>), g, b, 0, s, ps, sh.scene);
-        else hipLaunchKernelGGL((kTraceQueue<STACK, ANY, true, false, MIXED>), g, b, 0, s, ps, sh.scene);
-    } else {
-        if (sh.q_reg) hipLaunchKernelGGL((kTraceQueue<STACK, ANY, false, true, MIXED>), g, b, 0, s, ps, sh.scene);
-        else hipLaunchKernelGGL((kTraceQueue<STACK, ANY, false, false, MIXED>), g, b, 0, s, ps, sh.scene);
-    }
+    if (sh.refill == kTraceSplit) hipLaunchKernelGGL((kTraceGrouped<STACK, ANY, SPHERES, REG>), g, b, 0, s, ps);
+    else hipLaunchKernelGGL((kTraceQueue<STACK, ANY, SPHERES, REG>), g, b, 0, s, ps, sh.scene);
 }
 
 template <int STACK, bool ANY>
+void launchTraceQueueS(const PathState& ps, const LaunchShape& sh, hipStream_t s)
+{
+    if (sh.q_spheres) {
+        if (sh.q_reg) launchRefill<STACK, ANY, true, true>(ps, sh, s);
+        else launchRefill<STACK, ANY, true, false>(ps, sh, s);
+    } else {
+        if (sh.q_reg) launchRefill<STACK, ANY, false, true>(ps, sh, s);
+        else launchRefill<STACK, ANY, false, false>(ps, sh, s);
+    }
+}
+
+// The refill kernels' stack class is the shape's `stack`; the one-ray-per-lane
+// kernels' is the bounce's STACK (the shape's `pstack`).
+template <bool ANY>
 void launchTraceQueue(const PathState& ps, const LaunchShape& sh, hipStream_t s)
 {
-    if (sh.refill == kTraceSplit) launchTraceQueueM<STACK, ANY, true>(ps, sh, s);
-    else launchTraceQueueM<STACK, ANY, false>(ps, sh, s);
+    if (sh.stack <= kShortStack) launchTraceQueueS<kShortStack, ANY>(ps, sh, s);
+    else if (sh.stack <= 32) launchTraceQueueS<32, ANY>(ps, sh, s);
+    else launchTraceQueueS<64, ANY>(ps, sh, s);
 }
 
 template <int BLOCK, int STACK>
@@ -1027,7 +1146,7 @@ hipError_t bounce(const PathState& ps, const LaunchShape& sh, hipStream_t s)
     } else {
         if (sh.refill == kTraceSplit)
             hipLaunchKernelGGL((kExtend<BLOCK, STACK, true>), dim3(grid), dim3(BLOCK), 0, s, ps);
-        launchTraceQueue<STACK, false>(ps, sh, s);
+        launchTraceQueue<false>(ps, sh, s);
         hipLaunchKernelGGL((kCountBins<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     }
     hipLaunchKernelGGL(kPlanBins, dim3(1), dim3(1024), 0, s, ps, grid);
@@ -1039,7 +1158,7 @@ hipError_t bounce(const PathState& ps, const LaunchShape& sh, hipStream_t s)
         hipLaunchKernelGGL((kShadow<BLOCK, STACK, false>), dim3(grid), dim3(BLOCK), 0, s, ps);
     else if (sh.refill == kTraceSplit)
         hipLaunchKernelGGL((kShadow<BLOCK, STACK, true>), dim3(grid), dim3(BLOCK), 0, s, ps);
-    if (sh.refill != kTracePlain) launchTraceQueue<STACK, true>(ps, sh, s);
+    if (sh.refill != kTracePlain) launchTraceQueue<true>(ps, sh, s);
     hipLaunchKernelGGL((kFinish<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     return hipGetLastError();
 }
@@ -1058,9 +1177,10 @@ hipError_t bouncePair(const PathState& ps, const LaunchShape& sh, hipStream_t s)
 template <int BLOCK>
 hipError_t bounceB(const PathState& ps, const LaunchShape& sh, hipStream_t s)
 {
-    if (sh.stack <= kShortStack) return bouncePair<BLOCK, kShortStack>(ps, sh, s);
-    if (sh.stack <= 32) return bouncePair<BLOCK, 32>(ps, sh, s);
-    if (sh.stack <= 64) return bouncePair<BLOCK, 64>(ps, sh, s);
+    if (sh.stack > 64) return hipErrorInvalidValue;
+    if (sh.pstack <= kShortStack) return bouncePair<BLOCK, kShortStack>(ps, sh, s);
+    if (sh.pstack <= 32) return bouncePair<BLOCK, 32>(ps, sh, s);
+    if (sh.pstack <= 64) return bouncePair<BLOCK, 64>(ps, sh, s);
     return hipErrorInvalidValue;
 }
 
@@ -1072,8 +1192,9 @@ hipError_t launchBouncePair(const PathState& ps, const LaunchShape& sh, hipStrea
     case 64: return bounceB<64>(ps, sh, s);
     case 128: return bounceB<128>(ps, sh, s);
     case 256:
-        if (sh.stack <= kShortStack) return bouncePair<256, kShortStack>(ps, sh, s);
-        if (sh.stack <= 32) return bouncePair<256, 32>(ps, sh, s);
+        if (sh.stack > 64) return hipErrorInvalidValue;
+        if (sh.pstack <= kShortStack) return bouncePair<256, kShortStack>(ps, sh, s);
+        if (sh.pstack <= 32) return bouncePair<256, 32>(ps, sh, s);
         return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
     }

@@ -20,6 +20,7 @@ struct alignas(16) DevJob {
     uint32_t cancelled;
     uint32_t group;     // samples of one pixel that are consecutive in work order
     uint32_t estimator; // AB_ESTIMATOR_*
+    uint32_t tag;       // scene tag of the job's queue entries (see kSlotBits), 0 for none
     uint64_t seed;
     uint64_t work_count; // width * height * spa^2
     unsigned long long* accum; // 4 per pixel: r, g, b fixed point (2^-28), sample count
@@ -71,6 +72,16 @@ struct alignas(16) Counters {
     unsigned long long paths_done;
 };
 
+// Queue entries (active, shadow queues) hold a path's slot in their low
+// kSlotBits bits and, above them, the scene tag of its job: 0 for paths whose
+// rays go one per lane (kExtend, kShadow), or t in 1..kMaxTags-1 for paths of the
+// scene tag_scenes[t], whose rays kTraceGrouped traces a scene at a time. Copying
+// an entry from queue to queue keeps the tag; the termination and free lists
+// hold bare slots. Paths in flight are at most 2^kSlotBits.
+constexpr uint32_t kSlotBits = 26;
+constexpr uint32_t kSlotMask = (1u << kSlotBits) - 1u;
+constexpr uint32_t kMaxTags = 1u << (32 - kSlotBits);
+
 constexpr int kBins = 5; // miss, diffuse, conductor, dielectric, mix
 constexpr uint32_t kMaxGrid = 8192; // blocks of the grid-stride queue kernels
 
@@ -99,6 +110,7 @@ struct PathState {
     uint32_t* freeq;
     Counters* ctr;
     const DevJob* jobs;
+    const DevScene* tag_scenes; // kMaxTags entries: the scene of each tag (entry 0 unused)
     const DevSegment* segs; // ring of seg_mask + 1 entries
     uint32_t capacity;
     uint32_t seg_mask;
@@ -106,17 +118,18 @@ struct PathState {
 
 // How the extension and shadow rays are traced: one ray per lane (kernels
 // kExtend, kShadow); with lane refill (kTraceQueue) over the one scene of every
-// job; or split, the rays of mesh-only scenes with lane refill (each lane over
-// its job's scene) and the others one per lane.
+// job; or split, the rays of scenes with meshes with lane refill grouped by scene
+// (kTraceGrouped: a wave holds one scene at a time) and the others one per lane.
 enum TraceMode : uint32_t { kTracePlain = 0, kTraceOne = 1, kTraceSplit = 2 };
 
 struct LaunchShape {
     uint32_t block;        // 64, 128 or 256
     uint32_t grid;         // blocks of the grid-stride kernels
-    uint32_t stack;        // traversal stack depth (max over the jobs' scenes)
+    uint32_t stack;        // traversal stack class (max over the jobs' scenes)
+    uint32_t pstack{0};    // stack class of the one-ray-per-lane kernels (max over the scenes they trace)
     uint32_t refill{kTracePlain};
     uint32_t qgrid{0};     // one-wave blocks of the lane-refill kernels
-    uint32_t q_spheres{0}; // a scene kTraceQueue traces has spheres (the FP64 sphere test is compiled in)
+    uint32_t q_spheres{0}; // a scene the refill kernels trace has spheres (the FP64 sphere test is compiled in)
     uint32_t q_reg{0};     // such a scene's stack bound + 1 exceeds the class: TLAS leaf rest in a register
     DevScene scene{};      // kTraceOne: the scene
 };

@@ -337,9 +337,9 @@ ab_status ab_render_batch(ab_context* ctx, const ab_job_desc* jobs, uint32_t n_j
 
 /* Autotuner: times trial renders of `jobs` under candidate launch
  * configurations (block size, paths in flight, grid blocks) and sets the
- * context's configuration to the fastest. Trials keep the jobs' scenes, cameras
- * and resolutions and lower samples_per_axis so one trial takes at most 2^24
- * camera samples. AB_TUNE_BATCH times one ab_render_batch call over all jobs,
+ * context's configuration to the fastest. Trials keep the jobs' scenes, cameras,
+ * widths and samples per pixel and render the same share of every job's rows, so
+ * one trial takes about 2^26 camera samples. AB_TUNE_BATCH times one ab_render_batch call over all jobs,
  * AB_TUNE_PLAIN one ab_render call per job. `best` and `best_ms` (wall time of
  * the winning trial) may be NULL. Images never depend on the configuration, so
  * tuning changes only speed. No counterpart in the reference. */
