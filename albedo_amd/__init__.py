@@ -295,7 +295,8 @@ class Server:
         s = _abi.ServerStats()
         self.ctx.check(self.ctx.lib.ab_server_stats_get(self.ctx.handle, ctypes.byref(s)))
         return {"jobs_done": s.jobs_done, "jobs_rejected": s.jobs_rejected, "rays": s.rays, "busy_ms": s.busy_ms,
-                "p50_latency_ms": s.p50_latency_ms, "p99_latency_ms": s.p99_latency_ms}
+                "p50_latency_ms": s.p50_latency_ms, "p99_latency_ms": s.p99_latency_ms,
+                "hbm_charged": s.hbm_charged}
 
 
 def sums_resolve(sums: np.ndarray) -> np.ndarray:

@@ -121,7 +121,7 @@ class JobStatus(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [("jobs_done", ctypes.c_uint64), ("jobs_rejected", ctypes.c_uint64), ("rays", ctypes.c_uint64),
                 ("busy_ms", ctypes.c_double), ("p50_latency_ms", ctypes.c_double),
-                ("p99_latency_ms", ctypes.c_double)]
+                ("p99_latency_ms", ctypes.c_double), ("hbm_charged", ctypes.c_uint64)]
 
 
 # Every entry point of include/albedo_gpu.h: name -> (restype, argtypes).

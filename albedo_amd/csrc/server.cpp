@@ -41,6 +41,8 @@ struct ServerJob {
     void* dev{nullptr}; // accumulators, done counter and mean image (JobBuffers layout)
     uint32_t engine_index{0};
     uint64_t exposed{0}; // samples handed to the engine so far
+    unsigned long long work_end{0}; // global work index just past the job's last exposed sample
+    uint32_t drain_reads{0}; // cancelled: counter reads since generation passed work_end
     uint32_t flags{0};   // AB_JOB_*
     bool cancel_requested{false};
     std::vector<float> result;
@@ -83,7 +85,10 @@ struct Server {
     std::map<uint64_t, std::shared_ptr<ServerJob>> jobs;
     std::deque<uint64_t> pending;
     std::vector<uint64_t> running;
-    std::vector<void*> retired; // buffers of cancelled jobs, freed once the engine drains
+    // Cancelled jobs whose paths may still be in flight: their buffers are freed
+    // at cancellation (no kernel touches a cancelled job's accumulators), but
+    // their scenes stay among the scenes the bounces trace until they drain.
+    std::vector<std::shared_ptr<ServerJob>> draining;
     uint64_t used{0};
     uint64_t next_id{1};
     uint64_t jobs_done{0}, jobs_rejected{0};
@@ -97,8 +102,7 @@ struct Server {
     size_t rr{0};                       // SHARE: rotates which running job is served first
     uint32_t policy{AB_SCHED_SMALL_FIRST}; // guarded by mu
     unsigned long long* h_done{nullptr}; // pinned: finished-sample lanes of the running jobs
-    SceneSet live_set{};                // scenes of the running jobs (loop thread)
-    bool live_exact{false};             // no cancelled job's paths can be in flight
+    SceneSet live_set{};                // scenes of the running and draining jobs (loop thread)
     size_t h_done_jobs{0};              // jobs h_done has room for
 
     Server(ab_context* c, uint64_t b) : ctx(c), budget(b), engine(c) {}
@@ -182,6 +186,7 @@ ab_status Server::scheduleLocked()
             room = engine.exposeWork(j->engine_index, j->exposed, take);
             if (!room) return;
             j->exposed += take;
+            j->work_end = engine.workTotal();
             budget -= std::min(budget, take);
         };
         if (policy == AB_SCHED_SHARE) {
@@ -249,14 +254,19 @@ void Server::retireFinished()
             cancel = j.cancel_requested;
         }
         if (cancel) {
+            // Once the flag is on the device, later bounces end the job's paths
+            // at their next shading without touching its buffer, so the buffer
+            // goes now and the budget is charged only for what is allocated.
             (void)engine.cancelJob(j.engine_index);
+            (void)hipStreamSynchronize(engine.stream());
             std::lock_guard<std::mutex> lock(mu);
             j.state = kCancelled;
             j.finish_ms = msSince(ctx);
-            retired.push_back(j.dev);
+            (void)hipFree(j.dev);
             j.dev = nullptr;
             used -= j.footprint;
             running.erase(std::find(running.begin(), running.end(), j.id));
+            draining.push_back(live[i]);
             cv.notify_all();
             continue;
         }
@@ -309,14 +319,13 @@ void Server::loop()
                 thread_status = sst;
                 break;
             }
-            // Paths in flight belong to the running jobs, and to cancelled ones
-            // until the engine drains (their buffers wait in `retired`); a job
-            // retires only once all its samples have finished.
-            live_exact = retired.empty();
+            // Paths in flight belong to the running jobs (a job retires only once
+            // all its samples have finished) and to the draining cancelled ones.
             live_set = SceneSet{};
             for (uint64_t id : running) live_set.add(jobs[id]->prepared.scene);
+            for (const auto& j : draining) live_set.add(j->prepared.scene);
         }
-        if (engine.traceScenes(live_exact ? &live_set : nullptr) != AB_OK) {
+        if (engine.traceScenes(&live_set) != AB_OK) {
             std::lock_guard<std::mutex> lock(mu);
             thread_status = AB_DEVICE_ERROR;
             break;
@@ -332,12 +341,23 @@ void Server::loop()
         last_consumed = c.work_next - last.work_next;
         last = c;
         rays_now = c.extend_rays + c.shadow_rays;
+        // A cancelled job's paths end at their first shading after the flag: once
+        // generation has passed the job's last exposed sample at one counter read,
+        // every path of it was generated and shaded within the bounces before the
+        // next read. Two reads past it, it has no paths left.
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            draining.erase(std::remove_if(draining.begin(), draining.end(),
+                                          [&](const std::shared_ptr<ServerJob>& j) {
+                                              if (c.work_next >= j->work_end) ++j->drain_reads;
+                                              return j->drain_reads >= 2 || engine.idle(c);
+                                          }),
+                           draining.end());
+        }
         retireFinished();
         if (engine.idle(c)) {
             std::lock_guard<std::mutex> lock(mu);
             if (running.empty()) {
-                for (void* p : retired) (void)hipFree(p);
-                retired.clear();
                 if (engine.tableSize() > 0) {
                     rays_before += rays_now;
                     rays_now = 0;
@@ -356,8 +376,7 @@ void Server::loop()
         if (j.dev) (void)hipFree(j.dev);
         j.dev = nullptr;
     }
-    for (void* p : retired) (void)hipFree(p);
-    retired.clear();
+    draining.clear();
     running.clear();
     cv.notify_all();
 }
@@ -622,6 +641,7 @@ ab_status ab_server_stats_get(ab_context* ctx, ab_server_stats* out)
     std::lock_guard<std::mutex> lock(s->mu);
     out->jobs_done = s->jobs_done;
     out->jobs_rejected = s->jobs_rejected;
+    out->hbm_charged = s->used;
     out->rays = s->rays_before + s->rays_now;
     out->busy_ms = s->busy_ms;
     std::vector<double> l = s->latencies;

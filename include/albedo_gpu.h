@@ -399,6 +399,7 @@ typedef struct ab_server_stats {
     double busy_ms;
     double p50_latency_ms;
     double p99_latency_ms;
+    uint64_t hbm_charged;    /* bytes of the admitted jobs' device buffers, held against the budget */
 } ab_server_stats;
 ab_status ab_server_stats_get(ab_context* ctx, ab_server_stats* stats);
 

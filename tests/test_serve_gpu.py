@@ -84,6 +84,43 @@ def test_cancel_and_progressive_snapshots(ctx, scenes):
             srv.poll(big)
 
 
+def test_cancelled_jobs_release_their_memory_under_load(ctx, scenes):
+    # A long job keeps the server busy while two others are admitted and
+    # cancelled: their buffers go at once (charged bytes and device free memory
+    # both drop while the long job still runs), and two new jobs fill the budget
+    # they left and render as alone.
+    import torch
+    room, _ = scenes
+    size = (1920, 1080)
+    fp = Server.footprint(room, cam(16 / 9), *size, 1, 8)
+    with Server(ctx, 3 * fp) as srv:
+        long_job = srv.submit(room, cam(16 / 9), *size, 16, 8)
+        a = srv.submit(room, cam(16 / 9), *size, 16, 8)
+        b = srv.submit(room, cam(16 / 9), *size, 16, 8)
+        for _ in range(500):
+            if all(srv.poll(j, image=False)[1]["state"] == Server.RUNNING for j in (long_job, a, b)):
+                break
+            time.sleep(0.002)
+        assert srv.stats()["hbm_charged"] == 3 * fp
+        torch.cuda.synchronize()
+        free_before = torch.cuda.mem_get_info()[0]
+        srv.cancel(a)
+        srv.cancel(b)
+        assert srv.wait(a)[0] == _abi.AB_CANCELLED and srv.wait(b)[0] == _abi.AB_CANCELLED
+        assert srv.poll(long_job, image=False)[1]["state"] == Server.RUNNING
+        assert srv.stats()["hbm_charged"] == fp
+        assert torch.cuda.mem_get_info()[0] - free_before >= 1.5 * fp
+        c = srv.submit(room, cam(1.0), 64, 64, 2, 8)
+        d = srv.submit(room, cam(1.0), 48, 48, 2, 8, seed=3)
+        for j, w, seed in ((c, 64, 0), (d, 48, 3)):
+            code, _, img = srv.wait(j)
+            assert code == _abi.AB_OK
+            alone = room.render(cam(1.0), w, w, 2, 8, seed=seed)
+            assert np.array_equal(img.view(np.uint32), alone.view(np.uint32))
+        assert srv.wait(long_job)[0] == _abi.AB_OK
+        assert srv.stats()["hbm_charged"] == 0
+
+
 def test_progressive_snapshot_converges_to_final(ctx, scenes):
     room, _ = scenes
     with Server(ctx, 1 << 30) as srv:
