@@ -41,6 +41,22 @@ uint32_t gridFor(uint32_t configured, uint32_t capacity, uint32_t block)
 // Traversal stack class of the bounce kernels for a scene stack depth.
 uint32_t stackClass(uint32_t depth) { return depth <= kShortStack ? kShortStack : (depth <= 32 ? 32u : 64u); }
 
+// Stack class of the lane-refill kernels for the deepest scene they trace: the
+// smallest class that holds the bound and the rest of a TLAS leaf (one entry
+// more), else the smallest that holds the bound (the leaf's rest then waits in a
+// register).
+void chooseRefillStack(LaunchShape& s, uint32_t depth)
+{
+    const uint32_t classes[] = {kShortStack, kRefillStack24, 32u, 64u};
+    s.qstack = 64;
+    for (uint32_t k : classes)
+        if (depth + 1 <= k) {
+            s.qstack = k;
+            break;
+        }
+    s.q_reg = depth + 1 > s.qstack ? 1u : 0u;
+}
+
 // Block size and grid for the shape's one-ray-per-lane stack class.
 void settleBlocks(LaunchShape& s, const ab_launch_config& cfg, uint32_t capacity)
 {
@@ -77,14 +93,14 @@ void chooseTracing(LaunchShape& s, const SceneSet& set, uint32_t capacity, bool 
         s.refill = kTraceOne;
         s.scene = set.first;
         s.q_spheres = set.first.has_spheres ? 1u : 0u;
-        s.q_reg = set.first.stack_depth + 1 > s.stack ? 1u : 0u;
+        chooseRefillStack(s, set.first.stack_depth);
         s.qgrid = qgrid;
         s.pstack = kShortStack; // no ray goes one per lane
         settleBlocks(s, cfg, capacity);
     } else if (set.meshes) {
         s.refill = kTraceSplit;
         s.q_spheres = set.mesh_spheres ? 1u : 0u;
-        s.q_reg = set.deepest_meshes + 1 > s.stack ? 1u : 0u;
+        chooseRefillStack(s, set.deepest_meshes);
         s.qgrid = qgrid;
     }
 }
@@ -93,7 +109,7 @@ bool sameTracing(const LaunchShape& a, const LaunchShape& b)
 {
     if (a.refill != b.refill || a.pstack != b.pstack || a.block != b.block) return false;
     if (a.refill == kTracePlain) return true;
-    return a.qgrid == b.qgrid && a.q_spheres == b.q_spheres && a.q_reg == b.q_reg &&
+    return a.qgrid == b.qgrid && a.q_spheres == b.q_spheres && a.q_reg == b.q_reg && a.qstack == b.qstack &&
            std::memcmp(&a.scene, &b.scene, sizeof(DevScene)) == 0;
 }
 

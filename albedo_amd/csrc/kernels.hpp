@@ -22,6 +22,10 @@ constexpr uint32_t kMaxStackDepth = 256;
 // and registers become the limit (the ray-query kernels then ask the compiler
 // for 6-7 waves per SIMD); deeper stacks are LDS-bound.
 constexpr uint32_t kShortStack = 20;
+// A class between 20 and 32 for the path tracer's refill kernels: 6 KB of LDS per
+// wave leaves room for 6 waves per SIMD where 32 entries allow 5 (the
+// full-feature scene's bound is 22, one more with a TLAS leaf's rest on the stack).
+constexpr uint32_t kRefillStack24 = 24;
 
 // Maps block k of a G-block launch so each of the 8 XCDs (which receive blocks
 // round-robin) works on one contiguous range of logical blocks, keeping nearby

@@ -818,9 +818,16 @@ __device__ __forceinline__ RayF queuedRay(const PathState& ps, uint32_t slot)
     return RayF{o.x, o.y, o.z, 0.0f, d.x, d.y, d.z, ANY ? o.w : __builtin_huge_valf()};
 }
 
+// Waves per SIMD the refill kernels ask the compiler for: with the FP64 sphere
+// test 6 (80 VGPRs, which with the 24-entry stack class's 6 KB of LDS per wave
+// leaves room for 6 waves), without it 7 (72 VGPRs).
+template <bool SPHERES>
+constexpr int refillWaves() { return SPHERES ? 6 : 7; }
+
 // Every ray of the queue over one scene, passed as a kernel argument.
 template <int STACK, bool ANY, bool SPHERES, bool LEAF_IN_REG>
-__global__ __launch_bounds__(64) void kTraceQueue(PathState ps, DevScene sc)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(refillWaves<SPHERES>(), 8))) void kTraceQueue(
+    PathState ps, DevScene sc)
 {
     __shared__ int32_t stack[STACK * 64];
     const Counters* c = ps.ctr;
@@ -903,7 +910,8 @@ __device__ __forceinline__ DevScene sceneOfTag(const PathState& ps, uint32_t tag
 // scalar registers, as kTraceQueue does with its kernel argument, instead of a
 // scene per lane; the ownership and run checks read only the queue entries.
 template <int STACK, bool ANY, bool SPHERES, bool LEAF_IN_REG>
-__global__ __launch_bounds__(64) void kTraceGrouped(PathState ps)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(refillWaves<SPHERES>(), 8))) void kTraceGrouped(
+    PathState ps)
 {
     __shared__ int32_t stack[STACK * 64];
     const Counters* c = ps.ctr;
@@ -1126,13 +1134,14 @@ void launchTraceQueueS(const PathState& ps, const LaunchShape& sh, hipStream_t s
     }
 }
 
-// The refill kernels' stack class is the shape's `stack`; the one-ray-per-lane
+// The refill kernels' stack class is the shape's `qstack`; the one-ray-per-lane
 // kernels' is the bounce's STACK (the shape's `pstack`).
 template <bool ANY>
 void launchTraceQueue(const PathState& ps, const LaunchShape& sh, hipStream_t s)
 {
-    if (sh.stack <= kShortStack) launchTraceQueueS<kShortStack, ANY>(ps, sh, s);
-    else if (sh.stack <= 32) launchTraceQueueS<32, ANY>(ps, sh, s);
+    if (sh.qstack <= kShortStack) launchTraceQueueS<kShortStack, ANY>(ps, sh, s);
+    else if (sh.qstack <= kRefillStack24) launchTraceQueueS<kRefillStack24, ANY>(ps, sh, s);
+    else if (sh.qstack <= 32) launchTraceQueueS<32, ANY>(ps, sh, s);
     else launchTraceQueueS<64, ANY>(ps, sh, s);
 }
 

@@ -129,6 +129,7 @@ struct LaunchShape {
     uint32_t pstack{0};    // stack class of the one-ray-per-lane kernels (max over the scenes they trace)
     uint32_t refill{kTracePlain};
     uint32_t qgrid{0};     // one-wave blocks of the lane-refill kernels
+    uint32_t qstack{0};    // stack class of the lane-refill kernels (20, 24, 32 or 64 entries)
     uint32_t q_spheres{0}; // a scene the refill kernels trace has spheres (the FP64 sphere test is compiled in)
     uint32_t q_reg{0};     // such a scene's stack bound + 1 exceeds the class: TLAS leaf rest in a register
     DevScene scene{};      // kTraceOne: the scene
