@@ -58,6 +58,8 @@ def summarize(fetch_dir, write_dir):
     mean = sum(v["bytes"] for v in per_set.values()) / len(per_set)
     return {
         "kernel": KERNEL,
+        "command": "rocprofv3 --pmc FETCH_SIZE, then --pmc WRITE_SIZE, over python3 bench.py --trace-only "
+                   "--copy-probe --steps 2 --warmup 1 --no-cpu",
         "copy_dispatches": copy_ids,
         "copy_fetch_kib": copy_fetch, "copy_write_kib": copy_write,
         "fetch_scale": fetch_scale, "write_scale": write_scale,

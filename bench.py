@@ -146,11 +146,12 @@ def full_scene_leg(ctx, cpu=True):
     return out
 
 
-def batch_leg(ctx, n_jobs=16, full=False):
+def batch_leg(ctx, n_jobs=16, full=False, forms=("plain", "batched"), tune=True):
     """Mixed jobs (room / full-scene variants with 5..200 random spheres): one at a
-    time vs one batched launch, each with its autotuned configuration. Default:
-    16 jobs, 256x144..960x540, 4..64 spp; full: the 64-job mix of resolutions
-    256x144..1920x1080 and 4..256 spp."""
+    time vs one batched launch, each with its autotuned configuration (or, with
+    tune=False, the context's default one). Default: 16 jobs, 256x144..960x540,
+    4..64 spp; full: the 64-job mix of resolutions 256x144..1920x1080 and 4..256
+    spp. `forms` picks which of the two forms run (for profiling one alone)."""
     import albedo_amd
     from albedo_amd import workloads as W
     from albedo_amd.scene import CameraSettings
@@ -165,31 +166,39 @@ def batch_leg(ctx, n_jobs=16, full=False):
         jobs.append((ctx.scene(scene), CameraSettings(c.look_from, c.look_at, (0, 1, 0), 38.0, w / h), w, h, spa,
                      depth, 3, seed))
     base = ctx.launch_config()
-    # Each form runs with the launch configuration the autotuner picks for it.
-    cfg_plain, _ = ctx.autotune(jobs, batched=False)
-    print("batch: tuned the plain form", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    rays_plain = 0
-    for j in jobs:
-        _, st = j[0].render(*j[1:], stats=True)
-        rays_plain += st["camera_rays"] + st["extension_rays"] + st["shadow_rays"]
-    plain = time.perf_counter() - t0
-    print("batch: %d jobs one at a time in %.2f s" % (len(jobs), plain), file=sys.stderr, flush=True)
-    cfg_batch, _ = ctx.autotune(jobs, batched=True)
-    print("batch: tuned the batched form", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    _, st = albedo_amd.render_batch(ctx, jobs)
-    batched = time.perf_counter() - t0
-    print("batch: one batched launch in %.2f s" % batched, file=sys.stderr, flush=True)
-    ctx.set_launch_config(base.block_size, base.paths_in_flight, base.grid_blocks, base.use_graph)
-    rays_b = st["camera_rays"] + st["extension_rays"] + st["shadow_rays"]
+    default = {"block_size": base.block_size, "paths_in_flight": base.paths_in_flight,
+               "grid_blocks": base.grid_blocks, "use_graph": base.use_graph}
     samples = sum(j[2] * j[3] * j[4] ** 2 for j in jobs)
     sizes = "256x144..1920x1080, 4..256 spp" if full else "256x144..960x540, 4..64 spp"
-    return {"workload": "%d mixed jobs, %s, 5..200 random spheres" % (n_jobs, sizes),
-            "plain_s": round(plain, 3), "batched_s": round(batched, 3),
-            "plain_mrays_s": round(rays_plain / plain / 1e6, 1), "batched_mrays_s": round(rays_b / batched / 1e6, 1),
-            "batched_mpixel_spp_s": round(samples / batched / 1e6, 1),
-            "speedup": round(plain / batched, 3), "tuned_plain": cfg_plain, "tuned_batched": cfg_batch}
+    out = {"workload": "%d mixed jobs, %s, 5..200 random spheres" % (n_jobs, sizes), "samples": samples}
+    # Each form runs with the launch configuration the autotuner picks for it.
+    if "plain" in forms:
+        cfg_plain = ctx.autotune(jobs, batched=False)[0] if tune else default
+        print("batch: tuned the plain form", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        rays_plain = 0
+        for j in jobs:
+            _, st = j[0].render(*j[1:], stats=True)
+            rays_plain += st["camera_rays"] + st["extension_rays"] + st["shadow_rays"]
+        plain = time.perf_counter() - t0
+        print("batch: %d jobs one at a time in %.2f s" % (len(jobs), plain), file=sys.stderr, flush=True)
+        ctx.set_launch_config(base.block_size, base.paths_in_flight, base.grid_blocks, base.use_graph)
+        out.update({"plain_s": round(plain, 3), "plain_mrays_s": round(rays_plain / plain / 1e6, 1),
+                    "tuned_plain": cfg_plain})
+    if "batched" in forms:
+        cfg_batch = ctx.autotune(jobs, batched=True)[0] if tune else default
+        print("batch: tuned the batched form", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        _, st = albedo_amd.render_batch(ctx, jobs)
+        batched = time.perf_counter() - t0
+        print("batch: one batched launch in %.2f s" % batched, file=sys.stderr, flush=True)
+        ctx.set_launch_config(base.block_size, base.paths_in_flight, base.grid_blocks, base.use_graph)
+        rays_b = st["camera_rays"] + st["extension_rays"] + st["shadow_rays"]
+        out.update({"batched_s": round(batched, 3), "batched_mrays_s": round(rays_b / batched / 1e6, 1),
+                    "batched_mpixel_spp_s": round(samples / batched / 1e6, 1), "tuned_batched": cfg_batch})
+    if "plain_s" in out and "batched_s" in out:
+        out["speedup"] = round(out["plain_s"] / out["batched_s"], 3)
+    return out
 
 
 def build_leg(ctx):
@@ -339,6 +348,10 @@ def main():
                     help="legs after the closest-hit benchmark, comma separated: pt (sphere room), full "
                          "(full-feature scene), batch64 (the 64-job mix up to 1920x1080 at 256 spp), batch "
                          "(a 16-job mix up to 960x540 at 64 spp), serve, build")
+    ap.add_argument("--batch-forms", default="plain,batched",
+                    help="forms the batch legs run: plain (one job at a time), batched, or both")
+    ap.add_argument("--fixed-config", action="store_true",
+                    help="batch legs: the context's default launch configuration instead of the autotuner's")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy, the reference for counter-based traffic")
     args = ap.parse_args()
@@ -439,10 +452,11 @@ def main():
         out["path_tracing"] = path_tracing_leg(ctx, cpu=not args.no_cpu)
     if "full" in legs:
         out["full_scene"] = full_scene_leg(ctx, cpu=not args.no_cpu)
+    forms = tuple(args.batch_forms.split(","))
     if "batch" in legs:
-        out["batched"] = batch_leg(ctx)
+        out["batched"] = batch_leg(ctx, forms=forms, tune=not args.fixed_config)
     if "batch64" in legs:
-        out["batched_64"] = batch_leg(ctx, 64, full=True)
+        out["batched_64"] = batch_leg(ctx, 64, full=True, forms=forms, tune=not args.fixed_config)
     if "serve" in legs:
         out["served"] = serve_leg(ctx)
     if "build" in legs:
