@@ -339,8 +339,9 @@ ab_status ab_render_batch(ab_context* ctx, const ab_job_desc* jobs, uint32_t n_j
  * configurations (block size, paths in flight, grid blocks) and sets the
  * context's configuration to the fastest. Trials keep the jobs' scenes, cameras,
  * widths and samples per pixel and render the same share of every job's rows, so
- * one trial takes about 2^26 camera samples. AB_TUNE_BATCH times one ab_render_batch call over all jobs,
- * AB_TUNE_PLAIN one ab_render call per job. `best` and `best_ms` (wall time of
+ * one trial takes about 2^28 camera samples (all of them for smaller job lists).
+ * AB_TUNE_BATCH times one ab_render_batch call over all jobs, AB_TUNE_PLAIN one
+ * ab_render call per job. `best` and `best_ms` (wall time of
  * the winning trial) may be NULL. Images never depend on the configuration, so
  * tuning changes only speed. No counterpart in the reference. */
 #define AB_TUNE_PLAIN 0u
