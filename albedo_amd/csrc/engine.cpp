@@ -309,7 +309,7 @@ ab_status Engine::readCounters(Counters& out)
     // the next bounce, counted now.
     out.paths_done += out.n_term;
     out.shadow_rays += out.n_shadow;
-    out.n_free += out.n_term;
+    out.n_free += out.n_term - out.n_recycle;
     out.n_term = out.n_shadow = 0;
     return AB_OK;
 }

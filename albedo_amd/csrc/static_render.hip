@@ -116,7 +116,7 @@ __global__ void kStaticInit(PathState ps, DevJob job, unsigned long long* accum,
     const uint64_t i0 = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     for (uint64_t i = i0; i < accum_words; i += stride) accum[i] = 0ull;
     for (uint64_t i = i0; i < kDoneBytes / 8; i += stride) done[i] = 0ull;
-    for (uint64_t i = i0; i < ps.capacity; i += stride) ps.freeq[i] = uint32_t(i);
+    for (uint64_t i = i0; i < ps.capacity; i += stride) ps.freeq[i] = uint32_t(ps.capacity - 1 - i);
     if (i0 == 0) {
         *const_cast<DevJob*>(ps.jobs) = job;
         *const_cast<DevSegment*>(ps.segs) = DevSegment{0, 0, 0, 0};

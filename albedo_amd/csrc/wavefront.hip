@@ -43,19 +43,25 @@ __device__ __forceinline__ float powerHeuristic(float chosen, float other)
 }
 
 // The generation a bounce starts with, from the counters as the previous bounce
-// left them: the slots its finish freed (n_term, pushed above n_free) are free
-// again, and new samples fill free slots up to the work exposed. kGenerate and
-// kExtend compute it; kPlanBins applies it to the counters after both.
+// left them: new samples first take the n_recycle recycled slots and positions
+// (recycling is chosen only when the work left covers them, and work only
+// grows), then free slots, appended to the queue, up to the work exposed. The
+// slots the last finish freed (n_term - n_recycle, pushed above n_free) are free
+// again. kGenerate and the extension kernels compute it; kPlanBins applies it to
+// the counters after them.
 struct Generation {
-    uint32_t free_top; // free slots
-    uint32_t n_new;    // samples generated
+    uint32_t free_top;  // free slots
+    uint32_t n_new;     // samples generated
+    uint32_t n_recycle; // ... of which in recycled slots and positions
+    uint32_t n_extra;   // ... of which in free slots, appended to the queue
 };
 
 __device__ __forceinline__ Generation pendingGeneration(const Counters* c)
 {
-    const uint32_t free_top = c->n_free + c->n_term;
-    const unsigned long long left = c->work_total - c->work_next;
-    return {free_top, uint32_t(left < free_top ? left : free_top)};
+    const uint32_t free_top = c->n_free + (c->n_term - c->n_recycle);
+    const unsigned long long rest = c->work_total - c->work_next - c->n_recycle;
+    const uint32_t extra = uint32_t(rest < free_top ? rest : free_top);
+    return {free_top, c->n_recycle + extra, c->n_recycle, extra};
 }
 
 __device__ __forceinline__ uint32_t gridStart() { return blockIdx.x * blockDim.x + threadIdx.x; }
@@ -67,17 +73,27 @@ __device__ __forceinline__ uint32_t gridStride() { return gridDim.x * blockDim.x
 // PerspectiveCamera::generateRay, perspective_camera.cpp:74-87). A job's samples
 // are taken in passes of `group` samples per pixel: pass g covers samples
 // [g G, g G + G) of every pixel (the last pass fewer), pixel-major inside the
-// pass. Samples of one pixel are then neighbours in the queues, so kFinish can
-// sum them within a wave before touching the pixel.
+// pass. Samples of one pixel then take neighbouring queue positions (adjacent
+// when appended, the next recycled positions otherwise), so kFinish can often sum
+// them within a wave before touching the pixel.
 __global__ void kGenerate(PathState ps)
 {
     const Counters* c = ps.ctr;
     const Generation g = pendingGeneration(c);
-    const uint32_t n_new = g.n_new, free_top = g.free_top, act0 = c->n_active;
+    const uint32_t n_new = g.n_new, n_rec = g.n_recycle, free_top = g.free_top, act0 = c->n_active;
     const unsigned long long w0 = c->work_next;
     const uint32_t seg_lo = c->seg_lo, seg_hi = c->seg_hi;
     for (uint32_t i = gridStart(); i < n_new; i += gridStride()) {
-        const uint32_t slot = ps.freeq[free_top - 1 - i];
+        // A recycled terminated path's position and slot, or a free slot
+        // appended to the queue.
+        uint32_t pos, slot;
+        if (i < n_rec) {
+            pos = ps.termq[i];
+            slot = ps.active[pos] & kSlotMask;
+        } else {
+            pos = act0 + (i - n_rec);
+            slot = ps.freeq[free_top - 1 - (i - n_rec)];
+        }
         const unsigned long long w = w0 + i;
         const DevSegment& sg = findSegment(ps.segs, ps.seg_mask, seg_lo, seg_hi, w);
         const uint32_t j = sg.job;
@@ -115,7 +131,7 @@ __global__ void kGenerate(PathState ps)
         // No hit until the path is traced: a slot's previous path's hit must
         // never reach shading (it may index another scene's arrays).
         ps.hit[slot] = make_uint4(0x7f800000u, kMissId, kMissId, 0u);
-        ps.active[act0 + i] = slot | (job.tag << kSlotBits);
+        ps.active[pos] = slot | (job.tag << kSlotBits);
     }
 }
 
@@ -271,8 +287,9 @@ template <int BLOCK, int STACK, bool SPLIT>
 __global__ __launch_bounds__(BLOCK) void kExtend(PathState ps)
 {
     __shared__ int32_t stack[STACK * BLOCK];
-    // The queue holds the surviving paths and, after them, this bounce's new ones.
-    const uint32_t n = ps.ctr->n_active + pendingGeneration(ps.ctr).n_new;
+    // The queue holds the surviving (and recycled) paths and, after them, this
+    // bounce's new paths in free slots.
+    const uint32_t n = ps.ctr->n_active + pendingGeneration(ps.ctr).n_extra;
     WaveCounts<kBins> wc;
     for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
         const uint32_t i = base + threadIdx.x;
@@ -313,8 +330,9 @@ __global__ __launch_bounds__(1024) void kPlanBins(PathState ps, uint32_t grid)
         c.shadow_rays += c.n_shadow;
         c.n_term = 0;
         c.n_shadow = 0;
-        c.n_free = g.free_top - g.n_new;
-        c.n_active += g.n_new;
+        c.n_recycle = 0;
+        c.n_free = g.free_top - g.n_extra;
+        c.n_active += g.n_extra;
         c.work_next += g.n_new;
         c.camera_rays += g.n_new;
         c.extend_rays += c.n_active;
@@ -327,8 +345,8 @@ __global__ __launch_bounds__(1024) void kPlanBins(PathState ps, uint32_t grid)
 // Counting sort of the extended paths by material, so waves of the shade kernel
 // mostly run one material's code. Only the shade kernel reads in this order:
 // `binned` holds positions in `active`, and the queues the bounce produces keep
-// the order of `active` (generation order), so samples of one pixel stay
-// neighbours from bounce to bounce.
+// the order of `active`, so samples of one pixel stay near each other from
+// bounce to bounce.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void kScatterBins(PathState ps)
 {
@@ -711,11 +729,16 @@ __device__ __forceinline__ void countsBefore(const uint32_t* block_count, uint32
 }
 
 // Writes the shaded paths into the next extension queue and the shadow and
-// termination queues, keeping their order in `active`.
+// termination queues, keeping their order in `active`. When the work left
+// covers every path of the bounce, terminated paths keep their positions too
+// (recycling: the next bounce's first new samples take them over), so the next
+// extension queue is this one.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void kScatterShade(PathState ps)
 {
-    const uint32_t n = ps.ctr->n_in;
+    const Counters* cc = ps.ctr;
+    const uint32_t n = cc->n_in;
+    const bool recycle = cc->work_total - cc->work_next >= n;
     uint32_t run[3];
     countsBefore<BLOCK, 3>(ps.block_count, blockIdx.x, run);
     if (blockIdx.x == 0) {
@@ -725,9 +748,10 @@ __global__ __launch_bounds__(BLOCK) void kScatterShade(PathState ps)
         countsBefore<BLOCK, 3>(ps.block_count, gridDim.x, all);
         if (threadIdx.x == 0) {
             Counters* c = ps.ctr;
-            c->n_active = all[0];
+            c->n_active = recycle ? n : all[0];
             c->n_shadow = all[1];
             c->n_term = all[2];
+            c->n_recycle = recycle ? all[2] : 0u;
         }
     }
     for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
@@ -737,9 +761,10 @@ __global__ __launch_bounds__(BLOCK) void kScatterShade(PathState ps)
         blockRanks<BLOCK, 3>(flags, rank, total);
         if (i < n) {
             const uint32_t entry = ps.active[i]; // the slot and its scene tag
-            if (flags & kQueueActive) ps.active_next[run[0] + rank[0]] = entry;
+            if (recycle) ps.active_next[i] = entry;
+            else if (flags & kQueueActive) ps.active_next[run[0] + rank[0]] = entry;
             if (flags & kQueueShadow) ps.shadowq[run[1] + rank[1]] = entry;
-            if (flags & kQueueTerm) ps.termq[run[2] + rank[2]] = entry & kSlotMask;
+            if (flags & kQueueTerm) ps.termq[run[2] + rank[2]] = i;
         }
 #pragma unroll
         for (int q = 0; q < 3; ++q) run[q] += total[q];
@@ -831,7 +856,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(refillWaves<
 {
     __shared__ int32_t stack[STACK * 64];
     const Counters* c = ps.ctr;
-    const uint32_t n = ANY ? c->n_shadow : c->n_active + pendingGeneration(c).n_new;
+    const uint32_t n = ANY ? c->n_shadow : c->n_active + pendingGeneration(c).n_extra;
     const uint32_t per_lane = (n + 64 * gridDim.x - 1) / (64 * gridDim.x);
     const uint32_t slice = per_lane * 64;
     uint32_t next = xcdRemap(blockIdx.x, gridDim.x) * slice;
@@ -915,7 +940,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(refillWaves<
 {
     __shared__ int32_t stack[STACK * 64];
     const Counters* c = ps.ctr;
-    const uint32_t n = ANY ? c->n_shadow : c->n_active + pendingGeneration(c).n_new;
+    const uint32_t n = ANY ? c->n_shadow : c->n_active + pendingGeneration(c).n_extra;
     const uint32_t per_lane = (n + 64 * gridDim.x - 1) / (64 * gridDim.x);
     const uint32_t slice = per_lane * 64;
     uint32_t next = xcdRemap(blockIdx.x, gridDim.x) * slice;
@@ -981,7 +1006,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(refillWaves<
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void kCountBins(PathState ps)
 {
-    const uint32_t n = ps.ctr->n_active + pendingGeneration(ps.ctr).n_new;
+    const uint32_t n = ps.ctr->n_active + pendingGeneration(ps.ctr).n_extra;
     WaveCounts<kBins> wc;
     for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
         const uint32_t i = base + threadIdx.x;
@@ -999,9 +1024,10 @@ __device__ __forceinline__ unsigned long long toFixed(float v)
 }
 
 // Adds each finished path's radiance to its pixel (reference PixelBuffer::addSample,
-// image/pixel_buffer.cpp:29-35) and returns its slot to the free list.
-// Lanes that finish samples of the same pixel are usually neighbours (samples
-// of a pixel are generated together), so each wave first sums runs of equal
+// image/pixel_buffer.cpp:29-35) and returns its slot to the free list unless the
+// next bounce recycles it.
+// Lanes that finish samples of the same pixel are often neighbours (samples of
+// a pixel are generated together), so each wave first sums runs of equal
 // (job, pixel) keys and only the first lane of a run adds to the pixel: fewer
 // and far less contended atomics. Integer sums are exact, so the image is the
 // same as with one atomic add per sample.
@@ -1009,7 +1035,7 @@ template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void kFinish(PathState ps)
 {
     Counters* c = ps.ctr;
-    const uint32_t n = c->n_term, free0 = c->n_free;
+    const uint32_t n = c->n_term, free0 = c->n_free, n_rec = c->n_recycle;
     const uint32_t lane = __lane_id();
     for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
         const uint32_t i = base + threadIdx.x;
@@ -1017,7 +1043,7 @@ __global__ __launch_bounds__(BLOCK) void kFinish(PathState ps)
         bool counts = false;
         unsigned long long v0 = 0, v1 = 0, v2 = 0, cnt = 0;
         if (i < n) {
-            const uint32_t slot = ps.termq[i];
+            const uint32_t slot = ps.active[ps.termq[i]] & kSlotMask;
             j = __float_as_uint(ps.prev[slot].w);
             counts = !ps.jobs[j].cancelled;
             if (counts) {
@@ -1028,7 +1054,7 @@ __global__ __launch_bounds__(BLOCK) void kFinish(PathState ps)
                 cnt = 1;
                 pix = ps.pixel[slot];
             }
-            ps.freeq[free0 + i] = slot;
+            if (i >= n_rec) ps.freeq[free0 + (i - n_rec)] = slot;
         }
         const unsigned long long key = counts ? (static_cast<unsigned long long>(j) << 32) | pix : ~0ull;
         const unsigned long long up = __shfl_up(key, 1u);
@@ -1091,7 +1117,8 @@ __global__ void kResolve(const unsigned long long* accum, uint64_t npx, float* m
 __global__ void kReset(PathState ps, unsigned long long total, uint32_t n_segs)
 {
     const uint32_t i0 = gridStart();
-    for (uint32_t i = i0; i < ps.capacity; i += gridStride()) ps.freeq[i] = i;
+    // Stacked so the first generation takes slots 0, 1, 2, ... in queue order.
+    for (uint32_t i = i0; i < ps.capacity; i += gridStride()) ps.freeq[i] = ps.capacity - 1 - i;
     if (i0 == 0) {
         Counters* c = ps.ctr;
         *c = Counters{};

@@ -52,15 +52,24 @@ struct alignas(8) DevSegment {
 
 // Counters shared by the kernels of one engine.
 // n_term and n_shadow of a bounce are added to paths_done and shadow_rays (and
-// the terminated paths' slots to n_free) at the next bounce's kPlanBins; the
-// host adds what is still pending when it reads the block (readCounters).
+// the terminated paths' slots that are not recycled to n_free) at the next
+// bounce's kPlanBins; the host adds what is still pending when it reads the
+// block (readCounters).
+//
+// Recycling: while the work left covers every path of a bounce, a terminated
+// path keeps its place in the next active queue and the next bounce's first new
+// samples take over its slot and queue position (n_recycle of them, in
+// termination-queue order). The active queue then keeps one order for the whole
+// render -- ascending slots from the first generation on -- so neighbouring
+// lanes read neighbouring slots, instead of new paths taking freed slots that
+// lie anywhere in the state arrays.
 struct alignas(16) Counters {
-    uint32_t n_active;     // paths queued for extension
+    uint32_t n_active;     // paths queued for extension (with recycling: the kept positions too)
     uint32_t n_in;         // paths being shaded this bounce
     uint32_t n_shadow;
     uint32_t n_term;
     uint32_t n_free;
-    uint32_t pad0;
+    uint32_t n_recycle;    // terminated paths whose slot and position the next new samples take
     uint32_t bin_count[8]; // paths per material bin in the last bounce
     uint32_t seg_lo;       // live segments, logical ring indices [seg_lo, seg_hi)
     uint32_t seg_hi;
@@ -76,8 +85,8 @@ struct alignas(16) Counters {
 // kSlotBits bits and, above them, the scene tag of its job: 0 for paths whose
 // rays go one per lane (kExtend, kShadow), or t in 1..kMaxTags-1 for paths of the
 // scene tag_scenes[t], whose rays kTraceGrouped traces a scene at a time. Copying
-// an entry from queue to queue keeps the tag; the termination and free lists
-// hold bare slots. Paths in flight are at most 2^kSlotBits.
+// an entry from queue to queue keeps the tag; the free list holds bare slots and
+// the termination queue positions in `active`. Paths in flight are at most 2^kSlotBits.
 constexpr uint32_t kSlotBits = 26;
 constexpr uint32_t kSlotMask = (1u << kSlotBits) - 1u;
 constexpr uint32_t kMaxTags = 1u << (32 - kSlotBits);
@@ -102,12 +111,12 @@ struct PathState {
     float4* sh_org; // xyz origin, w tmax
     float4* sh_dir; // xyz direction
     float4* sh_c;   // xyz contribution if unoccluded
-    uint32_t* active;      // paths to extend, in the order they were generated
+    uint32_t* active;      // paths to extend (queue entries; see Counters on their order)
     uint32_t* active_next; // the next bounce's active queue (the two swap every bounce)
     uint32_t* binned;      // positions in `active`, grouped by the material hit
     uint32_t* shadowq;
-    uint32_t* termq;
-    uint32_t* freeq;
+    uint32_t* termq;       // positions in `active` of the terminated paths
+    uint32_t* freeq;       // free slots (a stack)
     Counters* ctr;
     const DevJob* jobs;
     const DevScene* tag_scenes; // kMaxTags entries: the scene of each tag (entry 0 unused)
