@@ -94,6 +94,7 @@ StaticLayout carve(void* base)
     s.block_count = reinterpret_cast<uint32_t*>(take(kBlockBytes));
     s.block_off = reinterpret_cast<uint32_t*>(take(kBlockBytes));
     s.ctr = reinterpret_cast<Counters*>(take(sizeof(Counters)));
+    // bytag and tag_count stay null: a render of one scene never traces split.
     L.job = reinterpret_cast<DevJob*>(take(sizeof(DevJob)));
     L.seg = reinterpret_cast<DevSegment*>(take(sizeof(DevSegment)));
     L.done = reinterpret_cast<unsigned long long*>(take(kDoneBytes));

@@ -273,6 +273,117 @@ __device__ __forceinline__ void blockRanks(uint32_t flags, uint32_t (&rank)[NQ],
     __syncthreads();
 }
 
+// ---------------------------------------------------------------- tag sort
+//
+// Split tracing (a queue of several scenes): the extension queue's positions
+// grouped by the scene tag of their entries, a counting sort over kTagGrid
+// blocks that each own a contiguous range of the queue. Untagged positions come
+// first, for the one-ray-per-lane kernels; each scene's rays then form one run
+// for kTraceGrouped, however the scenes' paths lie in the queue. Within a tag,
+// a block's positions keep their order in runs of at most a wave, so
+// neighbouring lanes still read neighbouring slots.
+
+// The queue range [lo, hi) of tag-sort block b.
+__device__ __forceinline__ void tagRange(uint32_t n, uint32_t b, uint32_t& lo, uint32_t& hi)
+{
+    const uint32_t per = ((n + kTagGrid - 1) / kTagGrid + kTagBlock - 1) / kTagBlock * kTagBlock;
+    lo = min(n, b * per);
+    hi = min(n, lo + per);
+}
+
+// For each distinct tag among the wave's valid lanes, in turn: calls
+// f(tag, lanes with it) on every lane (the mask is wave-uniform).
+template <class F>
+__device__ __forceinline__ void forEachTag(bool valid, uint32_t tag, F&& f)
+{
+    unsigned long long rest = __ballot(valid);
+    while (rest) {
+        const int first = __ffsll(static_cast<long long>(rest)) - 1;
+        const uint32_t t = __shfl(tag, first);
+        const unsigned long long m = __ballot(valid && tag == t);
+        f(t, m);
+        rest &= ~m;
+    }
+}
+
+__global__ __launch_bounds__(kTagBlock) void kTagCount(PathState ps)
+{
+    __shared__ uint32_t hist[kMaxTags];
+    const uint32_t n = ps.ctr->n_active + pendingGeneration(ps.ctr).n_extra;
+    uint32_t lo, hi;
+    tagRange(n, blockIdx.x, lo, hi);
+    if (threadIdx.x < kMaxTags) hist[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t lane = __lane_id();
+    for (uint32_t base = lo; base < hi; base += kTagBlock) {
+        const uint32_t i = base + threadIdx.x;
+        const bool valid = i < hi;
+        const uint32_t tag = valid ? ps.active[i] >> kSlotBits : 0u;
+        forEachTag(valid, tag, [&](uint32_t t, unsigned long long m) {
+            if (lane == uint32_t(__ffsll(static_cast<long long>(m)) - 1)) atomicAdd(&hist[t], uint32_t(__popcll(m)));
+        });
+    }
+    __syncthreads();
+    if (threadIdx.x < kMaxTags) ps.tag_count[threadIdx.x * kTagGrid + blockIdx.x] = hist[threadIdx.x];
+}
+
+// Exclusive scan of tag_count in tag-major order (tag t's blocks after every
+// block of the tags below it), in place; the untagged total to the counters.
+__global__ __launch_bounds__(1024) void kTagPlan(PathState ps)
+{
+    constexpr uint32_t kAll = kMaxTags * kTagGrid, kPer = kAll / 1024, kWaves = 1024 / 64;
+    static_assert(kAll % 1024 == 0 && kTagGrid % kPer == 0, "each thread scans whole runs of one tag");
+    __shared__ uint32_t wsum[kWaves];
+    const uint32_t t = threadIdx.x, lane = __lane_id(), wave = t >> 6;
+    uint32_t v[kPer];
+    uint32_t local = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        v[k] = ps.tag_count[t * kPer + k];
+        local += v[k];
+    }
+    uint32_t incl = local;
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t u = __shfl_up(incl, off);
+        if (lane >= off) incl += u;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    uint32_t run = incl - local;
+    for (uint32_t w = 0; w < wave; ++w) run += wsum[w];
+    // The thread whose entries start tag 1 begins at the untagged total.
+    if (t * kPer == kTagGrid) ps.ctr->n_untagged = run;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        ps.tag_count[t * kPer + k] = run;
+        run += v[k];
+    }
+}
+
+__global__ __launch_bounds__(kTagBlock) void kTagScatter(PathState ps)
+{
+    __shared__ uint32_t run[kMaxTags];
+    const uint32_t n = ps.ctr->n_active + pendingGeneration(ps.ctr).n_extra;
+    uint32_t lo, hi;
+    tagRange(n, blockIdx.x, lo, hi);
+    if (threadIdx.x < kMaxTags) run[threadIdx.x] = ps.tag_count[threadIdx.x * kTagGrid + blockIdx.x];
+    __syncthreads();
+    const uint32_t lane = __lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (uint32_t base = lo; base < hi; base += kTagBlock) {
+        const uint32_t i = base + threadIdx.x;
+        const bool valid = i < hi;
+        const uint32_t tag = valid ? ps.active[i] >> kSlotBits : 0u;
+        forEachTag(valid, tag, [&](uint32_t t, unsigned long long m) {
+            const int first = __ffsll(static_cast<long long>(m)) - 1;
+            uint32_t at = 0;
+            if (lane == uint32_t(first)) at = atomicAdd(&run[t], uint32_t(__popcll(m)));
+            at = __shfl(at, first);
+            if (valid && tag == t) ps.bytag[at + uint32_t(__popcll(m & below))] = i;
+        });
+    }
+}
+
 // ------------------------------------------------------------------ extend
 
 __device__ __forceinline__ uint32_t materialBin(const DevScene& sc, uint32_t obj)
@@ -281,22 +392,24 @@ __device__ __forceinline__ uint32_t materialBin(const DevScene& sc, uint32_t obj
     return sc.materials[sc.objects[obj].material].kind + 1;
 }
 
-// SPLIT: rays whose queue entries carry a scene tag are left to kTraceGrouped
-// (their keys too; the block counts then come from kCountBins).
+// SPLIT: only the untagged positions, which the tag sort put first in `bytag`;
+// rays whose queue entries carry a scene tag are left to kTraceGrouped (their
+// keys too; the block counts then come from kCountBins).
 template <int BLOCK, int STACK, bool SPLIT>
 __global__ __launch_bounds__(BLOCK) void kExtend(PathState ps)
 {
     __shared__ int32_t stack[STACK * BLOCK];
     // The queue holds the surviving (and recycled) paths and, after them, this
     // bounce's new paths in free slots.
-    const uint32_t n = ps.ctr->n_active + pendingGeneration(ps.ctr).n_extra;
+    const uint32_t n = SPLIT ? ps.ctr->n_untagged : ps.ctr->n_active + pendingGeneration(ps.ctr).n_extra;
     WaveCounts<kBins> wc;
     for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
-        const uint32_t i = base + threadIdx.x;
+        const uint32_t k = base + threadIdx.x;
+        const uint32_t i = SPLIT && k < n ? ps.bytag[k] : k;
         uint32_t flags = 0;
-        const uint32_t entry = i < n ? ps.active[i] : 0u;
+        const uint32_t entry = k < n ? ps.active[i] : 0u;
         const uint32_t slot = entry & kSlotMask;
-        if (i < n && !(SPLIT && (entry >> kSlotBits))) {
+        if (k < n) {
             const uint32_t j = __float_as_uint(ps.prev[slot].w);
             const float4 o = ps.org[slot], d = ps.dir[slot];
             const DevScene& sc = ps.jobs[j].scene;
@@ -684,15 +797,28 @@ __global__ __launch_bounds__(BLOCK) void kShade(PathState ps)
     }
 }
 
-// Counts the shaded paths per output queue and block, in `active` order.
+// The range [lo, hi) of queue order that block `b` of the shade-queue kernels
+// walks: contiguous, so the queues they write keep that order across blocks.
 template <int BLOCK>
+__device__ __forceinline__ void shadeRange(uint32_t n, uint32_t& lo, uint32_t& hi)
+{
+    const uint32_t per = ((n + gridDim.x - 1) / gridDim.x + BLOCK - 1) / BLOCK * BLOCK;
+    lo = min(n, blockIdx.x * per);
+    hi = min(n, lo + per);
+}
+
+// Counts the shaded paths per output queue and block, in `active` order (TAGGED:
+// in the tag sort's order, so the queues built from it come out sorted by tag).
+template <int BLOCK, bool TAGGED>
 __global__ __launch_bounds__(BLOCK) void kCountShade(PathState ps)
 {
     const uint32_t n = ps.ctr->n_in;
+    uint32_t lo, hi;
+    shadeRange<BLOCK>(n, lo, hi);
     WaveCounts<3> wc;
-    for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
-        const uint32_t i = base + threadIdx.x;
-        wc.add(i < n ? ps.key[i] : 0u);
+    for (uint32_t base = lo; base < hi; base += BLOCK) {
+        const uint32_t k = base + threadIdx.x;
+        wc.add(k < hi ? ps.key[TAGGED ? ps.bytag[k] : k] : 0u);
     }
     storeBlockCounts<BLOCK, 3>(wc, ps.block_count);
 }
@@ -729,11 +855,13 @@ __device__ __forceinline__ void countsBefore(const uint32_t* block_count, uint32
 }
 
 // Writes the shaded paths into the next extension queue and the shadow and
-// termination queues, keeping their order in `active`. When the work left
-// covers every path of the bounce, terminated paths keep their positions too
-// (recycling: the next bounce's first new samples take them over), so the next
-// extension queue is this one.
-template <int BLOCK>
+// termination queues, keeping their order in `active` (TAGGED: in the tag
+// sort's order; the shadow queue then holds its untagged entries first): block
+// b writes its range's items after those of the blocks before it. When
+// the work left covers every path of the bounce, terminated paths keep their
+// positions too (recycling: the next bounce's first new samples take them
+// over), so the next extension queue is this one.
+template <int BLOCK, bool TAGGED>
 __global__ __launch_bounds__(BLOCK) void kScatterShade(PathState ps)
 {
     const Counters* cc = ps.ctr;
@@ -752,14 +880,21 @@ __global__ __launch_bounds__(BLOCK) void kScatterShade(PathState ps)
             c->n_shadow = all[1];
             c->n_term = all[2];
             c->n_recycle = recycle ? all[2] : 0u;
+            if (TAGGED && cc->n_untagged >= n) c->sh_untagged = all[1];
         }
     }
-    for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
-        const uint32_t i = base + threadIdx.x;
-        const uint32_t flags = i < n ? ps.key[i] : 0u;
+    const uint32_t untagged = TAGGED ? cc->n_untagged : 0u;
+    uint32_t lo, hi;
+    shadeRange<BLOCK>(n, lo, hi);
+    for (uint32_t base = lo; base < hi; base += BLOCK) {
+        const uint32_t k = base + threadIdx.x;
+        const uint32_t i = TAGGED && k < hi ? ps.bytag[k] : k;
+        const uint32_t flags = k < hi ? ps.key[i] : 0u;
         uint32_t rank[3], total[3];
         blockRanks<BLOCK, 3>(flags, rank, total);
-        if (i < n) {
+        // The shadow entries of the untagged positions, which come first.
+        if (TAGGED && k == untagged && k < hi) ps.ctr->sh_untagged = run[1] + rank[1];
+        if (k < hi) {
             const uint32_t entry = ps.active[i]; // the slot and its scene tag
             if (recycle) ps.active_next[i] = entry;
             else if (flags & kQueueActive) ps.active_next[run[0] + rank[0]] = entry;
@@ -775,15 +910,15 @@ __global__ __launch_bounds__(BLOCK) void kScatterShade(PathState ps)
 
 // Occlusion of the deferred light samples (reference Scene::occluded,
 // scene/scene.cpp:157-165); unoccluded contributions join the path's radiance.
-// SPLIT: rays whose queue entries carry a scene tag are left to kTraceGrouped.
+// SPLIT: only the untagged entries, which come first in the shadow queue; rays
+// whose queue entries carry a scene tag are left to kTraceGrouped.
 template <int BLOCK, int STACK, bool SPLIT>
 __global__ __launch_bounds__(BLOCK) void kShadow(PathState ps)
 {
     __shared__ int32_t stack[STACK * BLOCK];
-    const uint32_t n = ps.ctr->n_shadow;
+    const uint32_t n = SPLIT ? ps.ctr->sh_untagged : ps.ctr->n_shadow;
     for (uint32_t i = gridStart(); i < n; i += gridStride()) {
         const uint32_t entry = ps.shadowq[i];
-        if (SPLIT && (entry >> kSlotBits)) continue;
         const uint32_t slot = entry & kSlotMask;
         const uint32_t j = __float_as_uint(ps.prev[slot].w);
         const float4 o = ps.sh_org[slot], d = ps.sh_dir[slot];
@@ -926,24 +1061,26 @@ __device__ __forceinline__ DevScene sceneOfTag(const PathState& ps, uint32_t tag
 }
 
 // In a queue of several scenes, the rays whose queue entries carry a scene tag
-// (the others go to kExtend / kShadow), one scene per wave at a time. Queues
-// keep the order in which samples were generated, so a job's rays form runs.
-// When every lane is idle the wave starts a run at its slice's next tagged ray,
-// skipping the other kernels' rays, and holds that run's scene in wave-uniform
-// registers; later refills take only the following rays of the same tag and
-// stop at the first ray of another. Traversal then reads one scene per wave from
-// scalar registers, as kTraceQueue does with its kernel argument, instead of a
-// scene per lane; the ownership and run checks read only the queue entries.
+// (the others go to kExtend / kShadow), one scene per wave at a time. The tag
+// sort orders the extension queue by tag (through `bytag`), and the shadow queue
+// is written in that order, so each scene's rays form one run after the
+// untagged ones. When every lane is idle the wave starts a run at its slice's
+// next ray and holds that run's scene in wave-uniform registers; later refills
+// take only the following rays of the same tag and stop at the first ray of
+// another. Traversal then reads one scene per wave from scalar registers, as
+// kTraceQueue does with its kernel argument, instead of a scene per lane.
 template <int STACK, bool ANY, bool SPHERES, bool LEAF_IN_REG>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(refillWaves<SPHERES>(), 8))) void kTraceGrouped(
     PathState ps)
 {
     __shared__ int32_t stack[STACK * 64];
     const Counters* c = ps.ctr;
+    // The tagged entries: [first, n) of the tag-sorted order.
+    const uint32_t first = ANY ? c->sh_untagged : c->n_untagged;
     const uint32_t n = ANY ? c->n_shadow : c->n_active + pendingGeneration(c).n_extra;
-    const uint32_t per_lane = (n + 64 * gridDim.x - 1) / (64 * gridDim.x);
+    const uint32_t per_lane = (n - first + 64 * gridDim.x - 1) / (64 * gridDim.x);
     const uint32_t slice = per_lane * 64;
-    uint32_t next = xcdRemap(blockIdx.x, gridDim.x) * slice;
+    uint32_t next = first + xcdRemap(blockIdx.x, gridDim.x) * slice;
     const uint32_t end = next + slice < n ? next + slice : n;
     const uint32_t lane = __lane_id();
     const LdsStack<64> stk{stack + threadIdx.x};
@@ -957,10 +1094,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(refillWaves<
         const unsigned long long free_lanes = __ballot(idle);
         const uint32_t n_free = uint32_t(__popcll(free_lanes));
         if (next < end && (n_free >= kQueueRefill || free_lanes == ~0ull)) {
-            // Lane i looks at window position next + i: its entry's scene tag
-            // (0: another kernel's ray).
+            // Lane i looks at window position next + i: its queue position, entry
+            // and the entry's scene tag (0: another kernel's ray).
             const uint32_t cnt = min(n_free, end - next);
-            const uint32_t w_entry = lane < cnt ? (ANY ? ps.shadowq[next + lane] : ps.active[next + lane]) : 0u;
+            const uint32_t w_pos = lane < cnt ? (ANY ? next + lane : ps.bytag[next + lane]) : 0u;
+            const uint32_t w_entry = lane < cnt ? (ANY ? ps.shadowq[w_pos] : ps.active[w_pos]) : 0u;
             const uint32_t w_tag = w_entry >> kSlotBits;
             uint32_t start = 0;
             bool fill = true;
@@ -980,10 +1118,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(refillWaves<
                 const unsigned long long other = __ballot(lane < cnt && lane >= start && w_tag != run_tag);
                 const uint32_t take = other ? uint32_t(__ffsll(static_cast<long long>(other))) - 1 : cnt;
                 const uint32_t k = start + uint32_t(__popcll(free_lanes & ((1ull << lane) - 1ull)));
+                const uint32_t k_entry = __shfl(w_entry, int(k & 63u)), k_pos = __shfl(w_pos, int(k & 63u));
                 if (idle && k < take) {
-                    slot = (ANY ? ps.shadowq[next + k] : ps.active[next + k]) & kSlotMask;
+                    slot = k_entry & kSlotMask;
                     q.begin(sc, queuedRay<ANY>(ps, slot));
-                    id = next + k;
+                    id = k_pos;
                 }
                 next += take;
             }
@@ -1176,23 +1315,33 @@ template <int BLOCK, int STACK>
 hipError_t bounce(const PathState& ps, const LaunchShape& sh, hipStream_t s)
 {
     const uint32_t grid = sh.grid;
+    const bool split = sh.refill == kTraceSplit;
     hipLaunchKernelGGL(kGenerate, dim3(grid), dim3(BLOCK), 0, s, ps);
+    if (split) {
+        hipLaunchKernelGGL(kTagCount, dim3(kTagGrid), dim3(kTagBlock), 0, s, ps);
+        hipLaunchKernelGGL(kTagPlan, dim3(1), dim3(1024), 0, s, ps);
+        hipLaunchKernelGGL(kTagScatter, dim3(kTagGrid), dim3(kTagBlock), 0, s, ps);
+    }
     if (sh.refill == kTracePlain) {
         hipLaunchKernelGGL((kExtend<BLOCK, STACK, false>), dim3(grid), dim3(BLOCK), 0, s, ps);
     } else {
-        if (sh.refill == kTraceSplit)
-            hipLaunchKernelGGL((kExtend<BLOCK, STACK, true>), dim3(grid), dim3(BLOCK), 0, s, ps);
+        if (split) hipLaunchKernelGGL((kExtend<BLOCK, STACK, true>), dim3(grid), dim3(BLOCK), 0, s, ps);
         launchTraceQueue<false>(ps, sh, s);
         hipLaunchKernelGGL((kCountBins<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     }
     hipLaunchKernelGGL(kPlanBins, dim3(1), dim3(1024), 0, s, ps, grid);
     hipLaunchKernelGGL((kScatterBins<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     hipLaunchKernelGGL((kShade<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
-    hipLaunchKernelGGL((kCountShade<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
-    hipLaunchKernelGGL((kScatterShade<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
+    if (split) {
+        hipLaunchKernelGGL((kCountShade<BLOCK, true>), dim3(grid), dim3(BLOCK), 0, s, ps);
+        hipLaunchKernelGGL((kScatterShade<BLOCK, true>), dim3(grid), dim3(BLOCK), 0, s, ps);
+    } else {
+        hipLaunchKernelGGL((kCountShade<BLOCK, false>), dim3(grid), dim3(BLOCK), 0, s, ps);
+        hipLaunchKernelGGL((kScatterShade<BLOCK, false>), dim3(grid), dim3(BLOCK), 0, s, ps);
+    }
     if (sh.refill == kTracePlain)
         hipLaunchKernelGGL((kShadow<BLOCK, STACK, false>), dim3(grid), dim3(BLOCK), 0, s, ps);
-    else if (sh.refill == kTraceSplit)
+    else if (split)
         hipLaunchKernelGGL((kShadow<BLOCK, STACK, true>), dim3(grid), dim3(BLOCK), 0, s, ps);
     if (sh.refill != kTracePlain) launchTraceQueue<true>(ps, sh, s);
     hipLaunchKernelGGL((kFinish<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);

@@ -79,6 +79,12 @@ struct alignas(16) Counters {
     unsigned long long extend_rays; // every closest-hit ray, camera rays included
     unsigned long long shadow_rays;
     unsigned long long paths_done;
+    // Split tracing: the extension queue's positions are sorted by scene tag
+    // (`bytag`), the untagged ones first; the shadow queue comes out in that
+    // order too. Untagged entries of each: [0, n_untagged), [0, sh_untagged).
+    uint32_t n_untagged;
+    uint32_t sh_untagged;
+    uint32_t pad1[2];
 };
 
 // Queue entries (active, shadow queues) hold a path's slot in their low
@@ -93,6 +99,10 @@ constexpr uint32_t kMaxTags = 1u << (32 - kSlotBits);
 
 constexpr int kBins = 5; // miss, diffuse, conductor, dielectric, mix
 constexpr uint32_t kMaxGrid = 8192; // blocks of the grid-stride queue kernels
+// The tag sort's kernels: kTagGrid blocks of kTagBlock threads, each block a
+// contiguous range of the queue.
+constexpr uint32_t kTagGrid = 512;
+constexpr uint32_t kTagBlock = 256;
 
 // Path state, structure of arrays over `capacity` slots.
 struct PathState {
@@ -117,6 +127,8 @@ struct PathState {
     uint32_t* shadowq;
     uint32_t* termq;       // positions in `active` of the terminated paths
     uint32_t* freeq;       // free slots (a stack)
+    uint32_t* bytag;       // split tracing: positions in `active`, sorted by scene tag
+    uint32_t* tag_count;   // split tracing: per tag and tag-sort block, entries, then first output index
     Counters* ctr;
     const DevJob* jobs;
     const DevScene* tag_scenes; // kMaxTags entries: the scene of each tag (entry 0 unused)

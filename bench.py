@@ -177,8 +177,10 @@ def batch_leg(ctx, n_jobs=16, full=False, forms=("plain", "batched"), tune=True)
         print("batch: tuned the plain form", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         rays_plain = 0
+        plain_imgs = []
         for j in jobs:
-            _, st = j[0].render(*j[1:], stats=True)
+            img, st = j[0].render(*j[1:], stats=True)
+            plain_imgs.append(img)
             rays_plain += st["camera_rays"] + st["extension_rays"] + st["shadow_rays"]
         plain = time.perf_counter() - t0
         print("batch: %d jobs one at a time in %.2f s" % (len(jobs), plain), file=sys.stderr, flush=True)
@@ -189,7 +191,7 @@ def batch_leg(ctx, n_jobs=16, full=False, forms=("plain", "batched"), tune=True)
         cfg_batch = ctx.autotune(jobs, batched=True)[0] if tune else default
         print("batch: tuned the batched form", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
-        _, st = albedo_amd.render_batch(ctx, jobs)
+        imgs, st = albedo_amd.render_batch(ctx, jobs)
         batched = time.perf_counter() - t0
         print("batch: one batched launch in %.2f s" % batched, file=sys.stderr, flush=True)
         ctx.set_launch_config(base.block_size, base.paths_in_flight, base.grid_blocks, base.use_graph)
@@ -198,6 +200,8 @@ def batch_leg(ctx, n_jobs=16, full=False, forms=("plain", "batched"), tune=True)
                     "batched_mpixel_spp_s": round(samples / batched / 1e6, 1), "tuned_batched": cfg_batch})
     if "plain_s" in out and "batched_s" in out:
         out["speedup"] = round(out["plain_s"] / out["batched_s"], 3)
+        same = sum(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(plain_imgs, imgs))
+        out["results_equal_alone"] = "%d/%d" % (same, len(jobs))
     return out
 
 

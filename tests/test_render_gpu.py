@@ -156,12 +156,16 @@ def test_split_tracing_equals_single_scene_tracing(ctx, room):
         assert np.array_equal(single.view(np.uint32), out.view(np.uint32))
 
 
-def test_split_tracing_with_full_queues_traces_every_ray(ctx, room):
+@pytest.mark.parametrize("block,grid", [(128, 0), (64, 8)])
+def test_split_tracing_with_full_queues_traces_every_ray(ctx, room, block, grid):
     """Split tracing with the queue full: 16 384 paths in flight, so each wave of
     the lane-refill kernel owns a slice of 6 x 64 queue entries, and runs of
     another scene's rays (which it skips) fill whole refills inside a slice. A
-    wave has to go on past such a refill to the rest of its slice: every job of
-    the batch equals its render alone."""
+    wave has to go on past such a refill to the rest of its slice. With a grid
+    of 8 blocks of 64 threads each block of the queue kernels walks 32 chunks of
+    the queue, and the shadow queue must still come out untagged entries first
+    for the split between the one-ray-per-lane and the grouped kernels. Every job
+    of the batch equals its render alone."""
     s, g, o = room
     ts = W.trefoil_scene()[0]
     ts.add_light(PointLight((2.0, 4.0, 9.0), (60.0, 60.0, 60.0)))
@@ -175,7 +179,7 @@ def test_split_tracing_with_full_queues_traces_every_ray(ctx, room):
             (tref, tcam, W.render_params(48, 32, 4, 6, 2, 45))]
     base = ctx.launch_config()
     try:
-        ctx.set_launch_config(128, 16384, 0, 1)
+        ctx.set_launch_config(block, 16384, grid, 1)
         outs, st = batch(ctx, jobs)
         singles = [gs.render(c, p.width, p.height, p.samples_per_axis, p.max_depth, p.roulette_start, p.seed)
                    for gs, c, p in jobs]
