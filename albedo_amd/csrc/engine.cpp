@@ -263,6 +263,7 @@ ab_status Engine::reserve(uint32_t capacity, uint32_t max_jobs, hipStream_t)
                     {reinterpret_cast<void**>(&p.termq), C * 4},     {reinterpret_cast<void**>(&p.freeq), C * 4},
                     {reinterpret_cast<void**>(&p.active_next), C * 4},
                     {reinterpret_cast<void**>(&p.bytag), C * 4},
+                    {reinterpret_cast<void**>(&p.slot_free), C},
                     {reinterpret_cast<void**>(&p.tag_count), size_t(kMaxTags) * kTagGrid * 4},
                     {reinterpret_cast<void**>(&p.block_count), size_t(kMaxGrid) * 8 * 4},
                     {reinterpret_cast<void**>(&p.block_off), size_t(kMaxGrid) * 8 * 4},

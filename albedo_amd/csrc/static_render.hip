@@ -35,10 +35,11 @@ constexpr size_t kSlotBytes8 = size_t(kStaticSlots) * 8;
 constexpr size_t kSlotBytes4 = size_t(kStaticSlots) * 4;
 constexpr size_t kBlockBytes = size_t(kMaxGrid) * 8 * 4;
 // org dir thr rad prev hit sh_org sh_dir sh_c (16 B), rng (8 B), pixel hit_b2 key
-// active active_next binned shadowq termq freeq (4 B), per-block counts and
-// offsets, counters, the job, the segment, the finished-sample counter and the
-// accumulators (32 B per pixel).
-constexpr size_t kPathBytes = 9 * align256(kSlotBytes16) + align256(kSlotBytes8) + 9 * align256(kSlotBytes4);
+// active active_next binned shadowq termq freeq (4 B), the slots' free flags
+// (1 B), per-block counts and offsets, counters, the job, the segment, the
+// finished-sample counter and the accumulators (32 B per pixel).
+constexpr size_t kPathBytes = 9 * align256(kSlotBytes16) + align256(kSlotBytes8) + 9 * align256(kSlotBytes4) +
+                              align256(kStaticSlots);
 constexpr size_t kFixedBytes = kPathBytes + 2 * align256(kBlockBytes) + align256(sizeof(Counters)) +
                                align256(sizeof(DevJob)) + align256(sizeof(DevSegment)) + align256(kDoneBytes);
 constexpr size_t kScratchBytes = kFixedBytes + size_t(kStaticPixels) * 32;
@@ -91,6 +92,7 @@ StaticLayout carve(void* base)
     s.shadowq = reinterpret_cast<uint32_t*>(take(kSlotBytes4));
     s.termq = reinterpret_cast<uint32_t*>(take(kSlotBytes4));
     s.freeq = reinterpret_cast<uint32_t*>(take(kSlotBytes4));
+    s.slot_free = reinterpret_cast<uint8_t*>(take(kStaticSlots));
     s.block_count = reinterpret_cast<uint32_t*>(take(kBlockBytes));
     s.block_off = reinterpret_cast<uint32_t*>(take(kBlockBytes));
     s.ctr = reinterpret_cast<Counters*>(take(sizeof(Counters)));
@@ -117,12 +119,16 @@ __global__ void kStaticInit(PathState ps, DevJob job, unsigned long long* accum,
     const uint64_t i0 = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     for (uint64_t i = i0; i < accum_words; i += stride) accum[i] = 0ull;
     for (uint64_t i = i0; i < kDoneBytes / 8; i += stride) done[i] = 0ull;
-    for (uint64_t i = i0; i < ps.capacity; i += stride) ps.freeq[i] = uint32_t(ps.capacity - 1 - i);
+    for (uint64_t i = i0; i < ps.capacity; i += stride) {
+        ps.freeq[i] = uint32_t(ps.capacity - 1 - i);
+        ps.slot_free[i] = 1;
+    }
     if (i0 == 0) {
         *const_cast<DevJob*>(ps.jobs) = job;
         *const_cast<DevSegment*>(ps.segs) = DevSegment{0, 0, 0, 0};
         Counters c{};
         c.n_free = ps.capacity;
+        c.free_sorted = 1;
         c.work_total = job.work_count;
         c.seg_lo = 0;
         c.seg_hi = 1;

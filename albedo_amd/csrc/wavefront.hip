@@ -67,6 +67,17 @@ __device__ __forceinline__ Generation pendingGeneration(const Counters* c)
 __device__ __forceinline__ uint32_t gridStart() { return blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ __forceinline__ uint32_t gridStride() { return gridDim.x * blockDim.x; }
 
+// The range [lo, hi) of an order over n items that this block walks when the
+// grid splits it into contiguous ranges (whole chunks of BLOCK), so that what
+// the blocks write in turn keeps that order.
+template <int BLOCK>
+__device__ __forceinline__ void blockSpan(uint32_t n, uint32_t& lo, uint32_t& hi)
+{
+    const uint32_t per = ((n + gridDim.x - 1) / gridDim.x + BLOCK - 1) / BLOCK * BLOCK;
+    lo = min(n, blockIdx.x * per);
+    hi = min(n, lo + per);
+}
+
 // ---------------------------------------------------------------- generate
 
 // Camera rays for the next n_new samples (reference render_image.cpp:27-44 and
@@ -93,6 +104,7 @@ __global__ void kGenerate(PathState ps)
         } else {
             pos = act0 + (i - n_rec);
             slot = ps.freeq[free_top - 1 - (i - n_rec)];
+            ps.slot_free[slot] = 0;
         }
         const unsigned long long w = w0 + i;
         const DevSegment& sg = findSegment(ps.segs, ps.seg_mask, seg_lo, seg_hi, w);
@@ -446,6 +458,9 @@ __global__ __launch_bounds__(1024) void kPlanBins(PathState ps, uint32_t grid)
         c.n_recycle = 0;
         c.n_free = g.free_top - g.n_extra;
         c.n_active += g.n_extra;
+        // Appending pops from a sorted stack (sorted this bounce if it was not),
+        // and what is left stays sorted.
+        if (g.n_extra > 0) c.free_sorted = 1;
         c.work_next += g.n_new;
         c.camera_rays += g.n_new;
         c.extend_rays += c.n_active;
@@ -797,16 +812,6 @@ __global__ __launch_bounds__(BLOCK) void kShade(PathState ps)
     }
 }
 
-// The range [lo, hi) of queue order that block `b` of the shade-queue kernels
-// walks: contiguous, so the queues they write keep that order across blocks.
-template <int BLOCK>
-__device__ __forceinline__ void shadeRange(uint32_t n, uint32_t& lo, uint32_t& hi)
-{
-    const uint32_t per = ((n + gridDim.x - 1) / gridDim.x + BLOCK - 1) / BLOCK * BLOCK;
-    lo = min(n, blockIdx.x * per);
-    hi = min(n, lo + per);
-}
-
 // Counts the shaded paths per output queue and block, in `active` order (TAGGED:
 // in the tag sort's order, so the queues built from it come out sorted by tag).
 template <int BLOCK, bool TAGGED>
@@ -814,7 +819,7 @@ __global__ __launch_bounds__(BLOCK) void kCountShade(PathState ps)
 {
     const uint32_t n = ps.ctr->n_in;
     uint32_t lo, hi;
-    shadeRange<BLOCK>(n, lo, hi);
+    blockSpan<BLOCK>(n, lo, hi);
     WaveCounts<3> wc;
     for (uint32_t base = lo; base < hi; base += BLOCK) {
         const uint32_t k = base + threadIdx.x;
@@ -885,7 +890,7 @@ __global__ __launch_bounds__(BLOCK) void kScatterShade(PathState ps)
     }
     const uint32_t untagged = TAGGED ? cc->n_untagged : 0u;
     uint32_t lo, hi;
-    shadeRange<BLOCK>(n, lo, hi);
+    blockSpan<BLOCK>(n, lo, hi);
     for (uint32_t base = lo; base < hi; base += BLOCK) {
         const uint32_t k = base + threadIdx.x;
         const uint32_t i = TAGGED && k < hi ? ps.bytag[k] : k;
@@ -903,6 +908,62 @@ __global__ __launch_bounds__(BLOCK) void kScatterShade(PathState ps)
         }
 #pragma unroll
         for (int q = 0; q < 3; ++q) run[q] += total[q];
+    }
+}
+
+// ---------------------------------------------------------- free-slot sort
+
+// The free stack rebuilt so that it pops slots in ascending order, before a
+// generation that appends paths to the queue: the appended paths then take
+// ascending slots, so neighbouring lanes read neighbouring slots. A finish
+// pushes its freed slots in queue order on top of older free slots, so
+// without this, paths appended after the queue ran short of work (a render's
+// last samples, a server between arrivals) would take slots spread over the
+// whole state arrays, and recycling would keep them there. Runs only when the
+// stack is unsorted and paths are appended; the slots' free flags give the set.
+__device__ __forceinline__ bool freeSortNeeded(const Counters* c)
+{
+    return !c->free_sorted && pendingGeneration(c).n_extra > 0;
+}
+
+// The slot range [lo, hi) of free-sort block b.
+__device__ __forceinline__ void freeRange(uint32_t cap, uint32_t b, uint32_t& lo, uint32_t& hi)
+{
+    const uint32_t per = ((cap + kFreeGrid - 1) / kFreeGrid + 255) / 256 * 256;
+    lo = min(cap, b * per);
+    hi = min(cap, lo + per);
+}
+
+__global__ __launch_bounds__(256) void kFreeCount(PathState ps)
+{
+    if (!freeSortNeeded(ps.ctr)) return;
+    uint32_t lo, hi;
+    freeRange(ps.capacity, blockIdx.x, lo, hi);
+    WaveCounts<1> wc;
+    for (uint32_t base = lo; base < hi; base += 256) {
+        const uint32_t s = base + threadIdx.x;
+        wc.add(s < hi && ps.slot_free[s] ? 1u : 0u);
+    }
+    storeBlockCounts<256, 1>(wc, ps.block_count);
+}
+
+__global__ __launch_bounds__(256) void kFreeScatter(PathState ps)
+{
+    if (!freeSortNeeded(ps.ctr)) return;
+    const uint32_t free_top = pendingGeneration(ps.ctr).free_top;
+    uint32_t run[1];
+    countsBefore<256, 1>(ps.block_count, blockIdx.x, run);
+    uint32_t lo, hi;
+    freeRange(ps.capacity, blockIdx.x, lo, hi);
+    for (uint32_t base = lo; base < hi; base += 256) {
+        const uint32_t s = base + threadIdx.x;
+        const uint32_t flag = s < hi && ps.slot_free[s] ? 1u : 0u;
+        uint32_t rank[1], total[1];
+        blockRanks<256, 1>(flag, rank, total);
+        // Stored from the top down, so the stack pops the lowest slot first.
+        const uint32_t r = run[0] + rank[0];
+        if (flag && r < free_top) ps.freeq[free_top - 1 - r] = s;
+        run[0] += total[0];
     }
 }
 
@@ -1176,6 +1237,8 @@ __global__ __launch_bounds__(BLOCK) void kFinish(PathState ps)
     Counters* c = ps.ctr;
     const uint32_t n = c->n_term, free0 = c->n_free, n_rec = c->n_recycle;
     const uint32_t lane = __lane_id();
+    // Slots pushed on the free stack unsort it.
+    if (blockIdx.x == 0 && threadIdx.x == 0 && n > n_rec) c->free_sorted = 0;
     for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
         const uint32_t i = base + threadIdx.x;
         uint32_t j = 0, pix = 0;
@@ -1193,7 +1256,10 @@ __global__ __launch_bounds__(BLOCK) void kFinish(PathState ps)
                 cnt = 1;
                 pix = ps.pixel[slot];
             }
-            if (i >= n_rec) ps.freeq[free0 + (i - n_rec)] = slot;
+            if (i >= n_rec) {
+                ps.freeq[free0 + (i - n_rec)] = slot;
+                ps.slot_free[slot] = 1;
+            }
         }
         const unsigned long long key = counts ? (static_cast<unsigned long long>(j) << 32) | pix : ~0ull;
         const unsigned long long up = __shfl_up(key, 1u);
@@ -1257,11 +1323,15 @@ __global__ void kReset(PathState ps, unsigned long long total, uint32_t n_segs)
 {
     const uint32_t i0 = gridStart();
     // Stacked so the first generation takes slots 0, 1, 2, ... in queue order.
-    for (uint32_t i = i0; i < ps.capacity; i += gridStride()) ps.freeq[i] = ps.capacity - 1 - i;
+    for (uint32_t i = i0; i < ps.capacity; i += gridStride()) {
+        ps.freeq[i] = ps.capacity - 1 - i;
+        ps.slot_free[i] = 1;
+    }
     if (i0 == 0) {
         Counters* c = ps.ctr;
         *c = Counters{};
         c->n_free = ps.capacity;
+        c->free_sorted = 1;
         c->work_total = total;
         c->seg_lo = 0;
         c->seg_hi = n_segs;
@@ -1316,6 +1386,8 @@ hipError_t bounce(const PathState& ps, const LaunchShape& sh, hipStream_t s)
 {
     const uint32_t grid = sh.grid;
     const bool split = sh.refill == kTraceSplit;
+    hipLaunchKernelGGL(kFreeCount, dim3(kFreeGrid), dim3(256), 0, s, ps);
+    hipLaunchKernelGGL(kFreeScatter, dim3(kFreeGrid), dim3(256), 0, s, ps);
     hipLaunchKernelGGL(kGenerate, dim3(grid), dim3(BLOCK), 0, s, ps);
     if (split) {
         hipLaunchKernelGGL(kTagCount, dim3(kTagGrid), dim3(kTagBlock), 0, s, ps);

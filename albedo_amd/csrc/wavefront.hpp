@@ -84,7 +84,10 @@ struct alignas(16) Counters {
     // order too. Untagged entries of each: [0, n_untagged), [0, sh_untagged).
     uint32_t n_untagged;
     uint32_t sh_untagged;
-    uint32_t pad1[2];
+    // The free stack pops slots in ascending order (set by a reset and by the
+    // free-slot sort, cleared when a finish pushes slots).
+    uint32_t free_sorted;
+    uint32_t pad1;
 };
 
 // Queue entries (active, shadow queues) hold a path's slot in their low
@@ -103,6 +106,9 @@ constexpr uint32_t kMaxGrid = 8192; // blocks of the grid-stride queue kernels
 // contiguous range of the queue.
 constexpr uint32_t kTagGrid = 512;
 constexpr uint32_t kTagBlock = 256;
+// The free-slot sort's kernels: kFreeGrid blocks of 256 threads, each a
+// contiguous range of the slots.
+constexpr uint32_t kFreeGrid = 256;
 
 // Path state, structure of arrays over `capacity` slots.
 struct PathState {
@@ -127,6 +133,7 @@ struct PathState {
     uint32_t* shadowq;
     uint32_t* termq;       // positions in `active` of the terminated paths
     uint32_t* freeq;       // free slots (a stack)
+    uint8_t* slot_free;    // per slot: 1 while it is on the free stack
     uint32_t* bytag;       // split tracing: positions in `active`, sorted by scene tag
     uint32_t* tag_count;   // split tracing: per tag and tag-sort block, entries, then first output index
     Counters* ctr;

@@ -261,14 +261,15 @@ def _fifo_latencies(arrivals, service):
     return np.array(out)
 
 
-def serve_leg(ctx, n_jobs=24, loads=(0.5, 1.0, 2.0), budget=1 << 30):
+def serve_leg(ctx, n_jobs=24, loads=(0.5, 1.0, 2.0), budget=1 << 30, runs=None):
     """Serving over a library of 8 HBM-resident scenes (the 1,310,720-triangle
     displaced icosphere x 64 instances on a checker floor, three sphere-room and
     four full-scene variants with 5..200 random spheres). Jobs: random scene,
     resolution 256x144..1920x1080, 4..256 spp. Poisson arrivals at rates swept as
     multiples of the plain path's measured capacity; admission by a 1 GiB HBM
     budget. The plain path (each job alone, one at a time, in arrival order) is
-    timed on the same jobs and replayed against the same arrival times."""
+    timed on the same jobs and replayed against the same arrival times. `runs`
+    (a list of (load, policy name)) replaces the default sweep."""
     import hashlib
     import albedo_amd
     from albedo_amd import _abi
@@ -303,8 +304,12 @@ def serve_leg(ctx, n_jobs=24, loads=(0.5, 1.0, 2.0), budget=1 << 30):
     print("serve: plain path %.2f s for %d jobs" % (sum(service), n_jobs), file=sys.stderr, flush=True)
     sweep = []
     names = {_abi.SCHED_FIFO: "fifo", _abi.SCHED_SHARE: "share", _abi.SCHED_SMALL_FIRST: "small_first"}
-    runs = [(load, _abi.SCHED_SMALL_FIRST) for load in loads]
-    runs += [(1.0, p) for p in (_abi.SCHED_FIFO, _abi.SCHED_SHARE)]
+    if runs is None:
+        runs = [(load, _abi.SCHED_SMALL_FIRST) for load in loads]
+        runs += [(1.0, p) for p in (_abi.SCHED_FIFO, _abi.SCHED_SHARE)]
+    else:
+        by_name = {v: k for k, v in names.items()}
+        runs = [(load, by_name[p]) for load, p in runs]
     for load, policy in runs:
         rate = load * capacity
         arrivals = W.poisson_arrivals(n_jobs, rate)
@@ -356,6 +361,9 @@ def main():
                     help="forms the batch legs run: plain (one job at a time), batched, or both")
     ap.add_argument("--fixed-config", action="store_true",
                     help="batch legs: the context's default launch configuration instead of the autotuner's")
+    ap.add_argument("--serve-runs", default=None,
+                    help="serve leg: comma-separated load:policy runs (policy fifo, share or small_first) "
+                         "instead of the default sweep")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy, the reference for counter-based traffic")
     args = ap.parse_args()
@@ -462,7 +470,10 @@ def main():
     if "batch64" in legs:
         out["batched_64"] = batch_leg(ctx, 64, full=True, forms=forms, tune=not args.fixed_config)
     if "serve" in legs:
-        out["served"] = serve_leg(ctx)
+        runs = None
+        if args.serve_runs:
+            runs = [(float(r.split(":")[0]), r.split(":")[1]) for r in args.serve_runs.split(",")]
+        out["served"] = serve_leg(ctx, runs=runs)
     if "build" in legs:
         out["bvh_build"] = build_leg(ctx)
     print(json.dumps(out))

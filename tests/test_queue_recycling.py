@@ -5,13 +5,17 @@ kScatterShade, kFinish, pendingGeneration) keep paths in `capacity` slots. While
 the work left covers every path of a bounce, a terminated path keeps its queue
 position, and the next bounce's first new samples take over its slot and
 position (in termination-queue order); otherwise survivors are compacted and
-freed slots go to a free stack, from which new samples are appended. The model
+freed slots go to a free stack, from which new samples are appended; before
+paths are appended from a stack that a finish has pushed to, the stack is
+rebuilt from the slots' free flags so it pops ascending slots. The model
 follows those steps over random termination patterns, capacities and work that
 arrives while the engine runs (as the server exposes it), and checks that no
 live slot is ever handed out twice, that the queue names each live path once,
 that occupied plus free slots always make up the capacity, and that every work
-index is generated exactly once. It also checks what recycling is for: until
-the work left no longer covers the queue, position p holds slot p. The planted
+index is generated exactly once. It also checks what recycling and the
+free-slot sort are for: until the work left no longer covers the queue,
+position p holds slot p, and the paths of one generation that are appended
+take ascending slots. The planted
 mistake recycles without checking that the work left covers the terminated
 paths, which leaves a position whose path never comes.
 """
@@ -20,12 +24,13 @@ import random
 import pytest
 
 
-def simulate(cap, arrivals, seed, check_work_left=True, max_bounces=100000):
+def simulate(cap, arrivals, seed, check_work_left=True, sort_free=True, max_bounces=100000):
     """arrivals: work_total values, the first at the start, the next every third
     bounce. Returns (bounces, identity_held)."""
     rnd = random.Random(seed)
     freeq = [cap - 1 - i for i in range(cap)]  # kReset: the first generation takes slots 0, 1, 2, ...
-    c = dict(n_active=0, n_term=0, n_free=cap, n_recycle=0, work_next=0, work_total=arrivals[0])
+    slot_free = [True] * cap
+    c = dict(n_active=0, n_term=0, n_free=cap, n_recycle=0, work_next=0, work_total=arrivals[0], free_sorted=True)
     active, active_next, termq = [None] * cap, [None] * cap, [None] * cap
     live = {}  # slot -> work index
     generated = []
@@ -42,6 +47,12 @@ def simulate(cap, arrivals, seed, check_work_left=True, max_bounces=100000):
             raise AssertionError("recycled positions left without work")
         extra = min(rest, free_top)
         n_new = c["n_recycle"] + extra
+        if sort_free and extra > 0 and not c["free_sorted"]:  # kFreeCount + kFreeScatter
+            free = [s for s in range(cap) if slot_free[s]]
+            assert len(free) == free_top
+            for r, s in enumerate(free):
+                freeq[free_top - 1 - r] = s
+        appended = []
         for i in range(n_new):
             if i < c["n_recycle"]:
                 pos = termq[i]
@@ -49,13 +60,18 @@ def simulate(cap, arrivals, seed, check_work_left=True, max_bounces=100000):
             else:
                 pos = c["n_active"] + (i - c["n_recycle"])
                 slot = freeq[free_top - 1 - (i - c["n_recycle"])]
+                slot_free[slot] = False
+                appended.append(slot)
             assert slot not in live, "a live slot handed out again"
             live[slot] = c["work_next"] + i
             generated.append(c["work_next"] + i)
             active[pos] = slot
+        assert appended == sorted(appended), "appended paths take ascending slots"
         # kPlanBins bookkeeping
         n = c["n_active"] + extra
         c.update(n_term=0, n_recycle=0, n_free=free_top - extra, n_active=n)
+        if extra > 0:
+            c["free_sorted"] = True
         c["work_next"] += n_new
         queue = active[:n]
         assert len(set(queue)) == n and all(s in live for s in queue)
@@ -79,11 +95,14 @@ def simulate(cap, arrivals, seed, check_work_left=True, max_bounces=100000):
         c["n_term"] = len(tpos)
         c["n_recycle"] = len(tpos) if recycle else 0
         # kFinish
+        if c["n_term"] > c["n_recycle"]:
+            c["free_sorted"] = False
         for k in range(c["n_term"]):
             slot = active[termq[k]]
             del live[slot]
             if k >= c["n_recycle"]:
                 freeq[c["n_free"] + (k - c["n_recycle"])] = slot
+                slot_free[slot] = True
         active, active_next = active_next, active
         if c["work_next"] == c["work_total"] and c["n_active"] == 0 and not later:
             assert sorted(generated) == list(range(c["work_total"])), "work generated once each"
@@ -118,4 +137,16 @@ def test_planted_recycling_without_the_work_check_fails():
             simulate(64, [300], seed, check_work_left=False)
         except AssertionError:
             failures += 1
+    assert failures > 0
+
+
+def test_planted_append_without_the_free_sort_scatters_slots():
+    # Work that arrives after the queue ran short: without the sort, paths
+    # appended from the free stack take its pushes' order.
+    failures = 0
+    for seed in range(30):
+        try:
+            simulate(64, [100, 400, 900, 1500], seed, sort_free=False)
+        except AssertionError as e:
+            failures += "ascending" in str(e)
     assert failures > 0
