@@ -53,16 +53,26 @@ def test_room_same_streams_match_fp64_estimator(room):
 
 
 def test_room_reference_sampling_at_4096_spp(room):
+    # The sphere room at 64x36 and 64x64 samples per pixel against the oracle in
+    # the reference's draw order (render/render_image.cpp:27-44). The bound is
+    # twice the difference between two independent oracle renders of the same
+    # view (seeds 11 and 12), over pixels and over 2x2 blocks; the mean within 2 %.
     s, g, o = room
-    w, h, n, depth = 32, 18, 64, 8
+    w, h, n, depth = 64, 36, 64, 8
     cam = CameraSettings(W.ROOM_CAMERA.look_from, W.ROOM_CAMERA.look_at, (0, 1, 0), 38.0, w / h)
     img = g.render(cam, w, h, n, max_depth=depth, roulette_start=3, seed=11)
     ref, _ = o.render(cam.to_abi(), W.render_params(w, h, n, depth, 3, 11), sample_mode=0)
-    err = rel_rmse(img, ref)
-    print("reference-sampling relative RMSE at 4096 spp: %.5f" % err)
-    # Two independent 4096-spp estimates of this scene differ by about 1-2 %
-    # relative RMSE (caustics through the glass spheres dominate); 5 % bounds it.
-    assert err < 0.05
+    ref2, _ = o.render(cam.to_abi(), W.render_params(w, h, n, depth, 3, 12), sample_mode=0)
+
+    def blocks(x):
+        return x.reshape(h // 2, 2, w // 2, 2, 3).mean((1, 3))
+
+    noise, noise_coarse = rel_rmse(ref2, ref), rel_rmse(blocks(ref2), blocks(ref))
+    full, coarse = rel_rmse(img, ref), rel_rmse(blocks(img), blocks(ref))
+    bias = float(img.mean() / ref.mean() - 1)
+    print("room at 4096 spp: relative RMSE %.4f (oracle seeds %.4f), 2x2 blocks %.4f (%.4f), mean %+.4f"
+          % (full, noise, coarse, noise_coarse, bias))
+    assert full < 2 * noise and coarse < 2 * noise_coarse and abs(bias) < 0.02
 
 
 def test_launch_configuration_does_not_change_the_image(ctx, room):
