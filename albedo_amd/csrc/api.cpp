@@ -123,25 +123,30 @@ ab_status ab_scene_create(ab_context* ctx, const ab_scene_desc* desc, ab_scene**
     if (stack > kMaxStackDepth)
         return fail(AB_INVALID_ARGUMENT, "ab_scene_create: BVH too deep for the traversal stack");
 
-    // One arena holds every array, each 256-byte aligned.
+    // One arena holds every array, each 256-byte aligned. Triangles take three
+    // copies (DevScene::tris): the built ones go into the third, whose
+    // coordinate order is the built (x, y, z), and a kernel rotates the others.
+    const size_t n_tris = hs.tris.size();
     struct Part {
         const void* src;
         size_t bytes; // copied from src
         size_t off;
-        size_t room;  // bytes the part occupies (>= bytes)
+        size_t room;  // bytes the part occupies (>= at + bytes)
+        size_t at;    // where in the part the copy goes
     };
     Part parts[] = {{hs.nodes.data(), hs.nodes.size() * sizeof(DevNode), 0,
-                     (hs.nodes.size() + device_nodes) * sizeof(DevNode)},
-                    {hs.tris.data(), hs.tris.size() * sizeof(DevTri), 0, 0},
-                    {hs.leafobj.data(), hs.leafobj.size() * sizeof(uint32_t), 0, 0},
-                    {hs.objects.data(), hs.objects.size() * sizeof(DevObject), 0, 0},
-                    {hs.spheres.data(), hs.spheres.size() * sizeof(double), 0, 0},
-                    {hs.shading.data(), hs.shading.size() * sizeof(DevTriShading), 0, 0},
-                    {hs.materials.data(), hs.materials.size() * sizeof(DevMaterial), 0, 0},
-                    {hs.lights.data(), hs.lights.size() * sizeof(DevLight), 0, 0}};
+                     (hs.nodes.size() + device_nodes) * sizeof(DevNode), 0},
+                    {hs.tris.data(), n_tris * sizeof(DevTri), 0, 3 * n_tris * sizeof(DevTri),
+                     2 * n_tris * sizeof(DevTri)},
+                    {hs.leafobj.data(), hs.leafobj.size() * sizeof(uint32_t), 0, 0, 0},
+                    {hs.objects.data(), hs.objects.size() * sizeof(DevObject), 0, 0, 0},
+                    {hs.spheres.data(), hs.spheres.size() * sizeof(double), 0, 0, 0},
+                    {hs.shading.data(), hs.shading.size() * sizeof(DevTriShading), 0, 0, 0},
+                    {hs.materials.data(), hs.materials.size() * sizeof(DevMaterial), 0, 0, 0},
+                    {hs.lights.data(), hs.lights.size() * sizeof(DevLight), 0, 0, 0}};
     size_t total = 0;
     for (Part& p : parts) {
-        p.room = std::max(p.room, p.bytes);
+        p.room = std::max(p.room, p.at + p.bytes);
         p.off = total;
         total += align256(std::max<size_t>(p.room, 16));
     }
@@ -151,23 +156,25 @@ ab_status ab_scene_create(ab_context* ctx, const ab_scene_desc* desc, ab_scene**
     // parts and the device-built nodes' room are never read before written.
     for (const Part& p : parts) {
         if (!p.bytes) continue;
-        if ((e = hipMemcpy(static_cast<unsigned char*>(arena) + p.off, p.src, p.bytes, hipMemcpyHostToDevice)) !=
-            hipSuccess) {
+        if ((e = hipMemcpy(static_cast<unsigned char*>(arena) + p.off + p.at, p.src, p.bytes,
+                           hipMemcpyHostToDevice)) != hipSuccess) {
             (void)hipFree(arena);
             return hipFail(e, "hipMemcpy(scene)");
         }
     }
     {
         // Device-built BLASes: nodes after the host's, triangles over their
-        // kept-order copies, in leaf order.
+        // kept-order copies, in leaf order; then the two rotated copies.
         auto* nodes = reinterpret_cast<DevNode*>(static_cast<unsigned char*>(arena) + parts[0].off);
         auto* tris = reinterpret_cast<DevTri*>(static_cast<unsigned char*>(arena) + parts[1].off);
+        DevTri* built = tris + 2 * n_tris;
         size_t base = hs.nodes.size();
         for (size_t i = 0; i < dblas.size() && e == hipSuccess; ++i) {
             const DeviceMesh& dm = hs.device_meshes[i];
-            e = dblas[i]->emit(nodes + base, uint32_t(base), tris + dm.tri_base, dm.tri_base, ctx->stream);
+            e = dblas[i]->emit(nodes + base, uint32_t(base), built + dm.tri_base, dm.tri_base, ctx->stream);
             base += dblas[i]->nodes();
         }
+        if (e == hipSuccess) e = launchRotateTris(tris, uint32_t(n_tris), ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         dblas.clear();
         if (e != hipSuccess) {
@@ -198,6 +205,7 @@ ab_status ab_scene_create(ab_context* ctx, const ab_scene_desc* desc, ab_scene**
     s->dev.num_lights = uint32_t(hs.lights.size());
     s->dev.sky_light = hs.sky_light;
     s->dev.stack_depth = stack;
+    s->dev.tri_count = uint32_t(n_tris);
     s->dev.has_spheres = 0;
     s->dev.has_meshes = 0;
     for (const DevObject& o : hs.objects) {

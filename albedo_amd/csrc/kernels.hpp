@@ -14,6 +14,10 @@ hipError_t launchTraceClosest(const DevScene& sc, const ab_ray* d_rays, ab_hit* 
 hipError_t launchTraceAny(const DevScene& sc, const ab_ray* d_rays, uint8_t* d_occluded, uint64_t n,
                           hipStream_t stream);
 
+// Fills copies 0 and 1 of a scene's triangles (DevScene::tris) from copy 2,
+// which holds them as built, coordinates in (x, y, z) order; n per copy.
+hipError_t launchRotateTris(DevTri* tris, uint32_t n, hipStream_t stream);
+
 // Largest stack depth the trace kernels support.
 constexpr uint32_t kMaxStackDepth = 256;
 

@@ -100,6 +100,9 @@ struct alignas(16) DevLight {
 // Pointers of one resident scene, passed by value to kernels.
 struct DevScene {
     const DevNode* nodes;
+    // Every triangle three times, in three copies of tri_count triangles: copy c
+    // holds the vertices' coordinates in the order (c + 1, c + 2, c) mod 3, the
+    // sheared-space order of a ray whose direction is largest along axis c.
     const DevTri* tris;
     const uint32_t* leafobj;
     const DevObject* objects;
@@ -114,6 +117,7 @@ struct DevScene {
     uint32_t stack_depth; // most traversal-stack entries a query holds (traceRay and RayQuery)
     uint32_t has_spheres; // any object is a sphere (else ray queries skip the FP64 test)
     uint32_t has_meshes;  // any object is a mesh instance (the path tracer then traces with lane refill)
+    uint32_t tri_count;   // triangles per copy in tris
 };
 
 } // namespace ab

@@ -12,6 +12,8 @@
 #include "kernels.hpp"
 #include "traverse.hpp"
 
+#include <algorithm>
+
 namespace ab {
 
 namespace {
@@ -122,7 +124,38 @@ hipError_t launchByStack(const DevScene& sc, const ab_ray* rays, ab_hit* hits, u
     return hipErrorInvalidValue;
 }
 
+// Copy c of a triangle: its vertices' coordinates in the order (c + 1, c + 2, c)
+// mod 3, the kept-face index kept beside the first vertex.
+__global__ void kRotateTris(DevTri* tris, uint32_t n)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const DevTri t = tris[2 * size_t(n) + i];
+        DevTri a = t, b = t;
+        const float* src[3] = {t.p0, t.p1, t.p2};
+        float* da[3] = {a.p0, a.p1, a.p2};
+        float* db[3] = {b.p0, b.p1, b.p2};
+        for (int v = 0; v < 3; ++v) {
+            da[v][0] = src[v][1]; // copy 0: (y, z, x)
+            da[v][1] = src[v][2];
+            da[v][2] = src[v][0];
+            db[v][0] = src[v][2]; // copy 1: (z, x, y)
+            db[v][1] = src[v][0];
+            db[v][2] = src[v][1];
+        }
+        tris[i] = a;
+        tris[size_t(n) + i] = b;
+    }
+}
+
 } // namespace
+
+hipError_t launchRotateTris(DevTri* tris, uint32_t n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    const uint32_t blocks = std::min<uint32_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(kRotateTris, dim3(blocks), dim3(256), 0, s, tris, n);
+    return hipGetLastError();
+}
 
 hipError_t launchTraceClosest(const DevScene& sc, const ab_ray* rays, ab_hit* hits, uint64_t n, hipStream_t s)
 {

@@ -78,11 +78,19 @@ __device__ __forceinline__ bool sphereHit(const double* s, const RayF& r, double
 }
 
 // The ray in the sheared space of Woop et al. 2013, set up once per (object
-// space) ray: permutation (kx, ky, kz) and shear (sx, sy, sz).
+// space) ray: kz, the axis of the direction's largest component, the axes
+// (kx, ky) = (kz + 1, kz + 2) mod 3, and the shear (sx, sy, sz). The triangle
+// test reads the copy of the triangles stored in that order (DevScene::tris),
+// so it selects no coordinates per lane. The reference's test
+// (geometry/triangle.cpp:95-131) also swaps kx and ky when the direction's
+// largest component is negative; the swap only negates the edge functions u,
+// v, w and their sum det, and negation is exact in floating point, so the
+// hit's t = (u az + v bz + w cz) / det and barycentrics v / det, w / det are
+// the same bits without it.
 struct ShearRay {
-    float ox, oy, oz; // origin permuted to (kx, ky, kz)
+    float ox, oy, oz; // origin in the order (kx, ky, kz)
     float sx, sy, sz;
-    int kx, ky, kz;
+    uint32_t kz;
     bool valid;       // |d|^2 is a normal number
 };
 
@@ -101,17 +109,10 @@ __device__ __forceinline__ ShearRay makeShear(float ox, float oy, float oz, floa
     int kz;
     if (ax > ay) kz = ax > az ? 0 : 2;
     else kz = ay > az ? 1 : 2;
-    int kx = kz == 2 ? 0 : kz + 1;
-    int ky = kx == 2 ? 0 : kx + 1;
+    const int kx = kz == 2 ? 0 : kz + 1;
+    const int ky = kx == 2 ? 0 : kx + 1;
     const float dzv = pick(dx, dy, dz, kz);
-    if (dzv < 0) {
-        const int t = kx;
-        kx = ky;
-        ky = t;
-    }
-    s.kx = kx;
-    s.ky = ky;
-    s.kz = kz;
+    s.kz = uint32_t(kz);
     s.sx = pick(dx, dy, dz, kx) / dzv;
     s.sy = pick(dx, dy, dz, ky) / dzv;
     s.sz = 1.0f / dzv;
@@ -122,22 +123,24 @@ __device__ __forceinline__ ShearRay makeShear(float ox, float oy, float oz, floa
 }
 
 // Watertight triangle test (reference geometry/triangle.cpp:132-176) in FP32,
-// with the edge functions recomputed in FP64 when one is exactly zero. The
-// barycentrics are v / det and w / det; callers divide only for a hit they keep.
+// with the edge functions recomputed in FP64 when one is exactly zero, on a
+// triangle from the copy of the ray's kz (coordinates already in the order
+// (kx, ky, kz)). The barycentrics are v / det and w / det; callers divide only
+// for a hit they keep.
 __device__ __forceinline__ bool triHit(const ShearRay& s, const float4& v0, const float4& v1, const float4& v2,
                                        float tmin, float tmax, float& t_out, float& v_out, float& w_out,
                                        float& det_out)
 {
 #pragma clang fp contract(off)
-    const float az_ = pick(v0.x, v0.y, v0.z, s.kz) - s.oz;
-    const float bz_ = pick(v1.x, v1.y, v1.z, s.kz) - s.oz;
-    const float cz_ = pick(v2.x, v2.y, v2.z, s.kz) - s.oz;
-    const float ax = (pick(v0.x, v0.y, v0.z, s.kx) - s.ox) - s.sx * az_;
-    const float ay = (pick(v0.x, v0.y, v0.z, s.ky) - s.oy) - s.sy * az_;
-    const float bx = (pick(v1.x, v1.y, v1.z, s.kx) - s.ox) - s.sx * bz_;
-    const float by = (pick(v1.x, v1.y, v1.z, s.ky) - s.oy) - s.sy * bz_;
-    const float cx = (pick(v2.x, v2.y, v2.z, s.kx) - s.ox) - s.sx * cz_;
-    const float cy = (pick(v2.x, v2.y, v2.z, s.ky) - s.oy) - s.sy * cz_;
+    const float az_ = v0.z - s.oz;
+    const float bz_ = v1.z - s.oz;
+    const float cz_ = v2.z - s.oz;
+    const float ax = (v0.x - s.ox) - s.sx * az_;
+    const float ay = (v0.y - s.oy) - s.sy * az_;
+    const float bx = (v1.x - s.ox) - s.sx * bz_;
+    const float by = (v1.y - s.oy) - s.sy * bz_;
+    const float cx = (v2.x - s.ox) - s.sx * cz_;
+    const float cy = (v2.y - s.oy) - s.sy * cz_;
     float u = cx * by - cy * bx;
     float v = ax * cy - ay * cx;
     float w = bx * ay - by * ax;
@@ -280,7 +283,8 @@ __device__ __forceinline__ bool traverseBlas(const DevScene& sc, int32_t root, u
             const uint32_t enc = uint32_t(~ref);
             const uint32_t first = enc >> kLeafCountBits, count = enc & ((1u << kLeafCountBits) - 1);
             for (uint32_t i = 0; i < count; ++i) {
-                const float4* tp = reinterpret_cast<const float4*>(sc.tris + first + i);
+                const float4* tp =
+                    reinterpret_cast<const float4*>(sc.tris + size_t(sh.kz) * sc.tri_count + first + i);
                 const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
                 float t, v, w, det;
                 if (triHit(sh, v0, v1, v2, r.tmin, r.tmax, t, v, w, det)) {
@@ -494,7 +498,8 @@ struct RayQuery {
             // between lanes at leaves and lanes at nodes.
             const uint32_t n = count < kLeafTrisPerStep ? count : kLeafTrisPerStep;
             for (uint32_t i = 0; i < n; ++i) {
-                const float4* tp = reinterpret_cast<const float4*>(sc.tris + first + i);
+                const float4* tp =
+                    reinterpret_cast<const float4*>(sc.tris + size_t(sh.kz) * sc.tri_count + first + i);
                 const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
                 float t, v, w, det;
                 if (triHit(sh, v0, v1, v2, r.tmin, r.tmax, t, v, w, det)) {

@@ -1118,6 +1118,7 @@ __device__ __forceinline__ DevScene sceneOfTag(const PathState& ps, uint32_t tag
     u.stack_depth = waveUniform(s.stack_depth);
     u.has_spheres = waveUniform(s.has_spheres);
     u.has_meshes = waveUniform(s.has_meshes);
+    u.tri_count = waveUniform(s.tri_count);
     return u;
 }
 
