@@ -92,6 +92,8 @@ struct ShearRay {
     float sx, sy, sz;
     uint32_t kz;
     bool valid;       // |d|^2 is a normal number
+    const DevTri* tris; // the copy of the triangles in this order (set with useCopy)
+    __device__ __forceinline__ void useCopy(const DevScene& sc) { tris = sc.tris + size_t(kz) * sc.tri_count; }
 };
 
 __device__ __forceinline__ float pick(float x, float y, float z, int k)
@@ -257,8 +259,9 @@ template <int BLOCK, bool ANY>
 __device__ __forceinline__ bool traverseBlas(const DevScene& sc, int32_t root, uint32_t object, const RayF& r,
                                              HitF& best, LdsStack<BLOCK> stk, int sp0)
 {
-    const ShearRay sh = makeShear(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz);
+    ShearRay sh = makeShear(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz);
     if (!sh.valid) return false;
+    sh.useCopy(sc);
     const BoxRay br = makeBoxRay(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz);
     int sp = sp0;
     int32_t ref = root;
@@ -283,8 +286,7 @@ __device__ __forceinline__ bool traverseBlas(const DevScene& sc, int32_t root, u
             const uint32_t enc = uint32_t(~ref);
             const uint32_t first = enc >> kLeafCountBits, count = enc & ((1u << kLeafCountBits) - 1);
             for (uint32_t i = 0; i < count; ++i) {
-                const float4* tp =
-                    reinterpret_cast<const float4*>(sc.tris + size_t(sh.kz) * sc.tri_count + first + i);
+                const float4* tp = reinterpret_cast<const float4*>(sh.tris + first + i);
                 const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
                 float t, v, w, det;
                 if (triHit(sh, v0, v1, v2, r.tmin, r.tmax, t, v, w, det)) {
@@ -498,8 +500,7 @@ struct RayQuery {
             // between lanes at leaves and lanes at nodes.
             const uint32_t n = count < kLeafTrisPerStep ? count : kLeafTrisPerStep;
             for (uint32_t i = 0; i < n; ++i) {
-                const float4* tp =
-                    reinterpret_cast<const float4*>(sc.tris + size_t(sh.kz) * sc.tri_count + first + i);
+                const float4* tp = reinterpret_cast<const float4*>(sh.tris + first + i);
                 const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
                 float t, v, w, det;
                 if (triHit(sh, v0, v1, v2, r.tmin, r.tmax, t, v, w, det)) {
@@ -568,6 +569,7 @@ struct RayQuery {
             pop(stk);
             return;
         }
+        sh.useCopy(sc);
         box = makeBoxRay(ro.ox, ro.oy, ro.oz, ro.dx, ro.dy, ro.dz);
         obj = o;
         in_blas = true;
