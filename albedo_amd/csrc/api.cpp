@@ -112,8 +112,12 @@ ab_status ab_scene_create(ab_context* ctx, const ab_scene_desc* desc, ab_scene**
         device_nodes += b->nodes();
         dblas.push_back(std::move(b));
     }
-    if (hs.nodes.size() + device_nodes >= size_t(INT32_MAX))
-        return fail(AB_INVALID_ARGUMENT, "ab_scene_create: too many BVH nodes");
+    if (hs.tris.size() > size_t(kMaxTriangles))
+        return fail(AB_INVALID_ARGUMENT, "ab_scene_create: too many triangles (at most " +
+                                             std::to_string(kMaxTriangles) + ")");
+    if (hs.nodes.size() + device_nodes >= size_t(kMaxNodes))
+        return fail(AB_INVALID_ARGUMENT, "ab_scene_create: too many BVH nodes (at most " +
+                                             std::to_string(kMaxNodes) + ")");
     // A traversal pushes one entry per interior node on its current root-to-leaf
     // path, so it needs at most (levels - 1) entries per tree; a BLAS traversal
     // starts on top of the TLAS entries pending at the instance's leaf (the rest of

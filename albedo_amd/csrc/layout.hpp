@@ -98,6 +98,13 @@ struct alignas(16) DevLight {
 };
 
 // Pointers of one resident scene, passed by value to kernels.
+// Nodes of one scene's BVHs (TLAS and every BLAS together): their bytes must
+// fit a 32-bit offset.
+constexpr uint32_t kMaxNodes = 1u << 26;
+static_assert(sizeof(DevNode) * uint64_t(kMaxNodes) <= (uint64_t(1) << 32), "node offsets fit 32 bits");
+// Triangles of one scene: the three copies' bytes must fit a 32-bit offset.
+constexpr uint32_t kMaxTriangles = uint32_t((uint64_t(1) << 32) / (3 * 48));
+
 struct DevScene {
     const DevNode* nodes;
     // Every triangle three times, in three copies of tri_count triangles: copy c

@@ -92,9 +92,17 @@ struct ShearRay {
     float sx, sy, sz;
     uint32_t kz;
     bool valid;       // |d|^2 is a normal number
-    const DevTri* tris; // the copy of the triangles in this order (set with useCopy)
-    __device__ __forceinline__ void useCopy(const DevScene& sc) { tris = sc.tris + size_t(kz) * sc.tri_count; }
+    uint32_t tri0;    // index of the first triangle of the copy in this order (set with useCopy)
+    __device__ __forceinline__ void useCopy(const DevScene& sc) { tri0 = kz * sc.tri_count; }
 };
+
+// Triangle i of the ray's copy; the byte offset fits 32 bits (kMaxTriangles), so
+// where the scene's arrays are wave-uniform the loads take a scalar base.
+__device__ __forceinline__ const float4* triOf(const DevScene& sc, const ShearRay& sh, uint32_t i)
+{
+    return reinterpret_cast<const float4*>(reinterpret_cast<const char*>(sc.tris) +
+                                           (sh.tri0 + i) * uint32_t(sizeof(DevTri)));
+}
 
 __device__ __forceinline__ float pick(float x, float y, float z, int k)
 {
@@ -224,9 +232,13 @@ __device__ __forceinline__ void testChildren(const DevNode& n, const BoxRay& r, 
     n1 = tn1;
 }
 
+// Node i of a scene (i >= 0, fewer than kMaxNodes): its byte offset fits 32
+// bits, so where `nodes` is wave-uniform the loads take a scalar base and a
+// 32-bit vector offset instead of a 64-bit address per lane.
 __device__ __forceinline__ DevNode loadNode(const DevNode* nodes, int32_t i)
 {
-    const float4* p = reinterpret_cast<const float4*>(nodes + i);
+    const float4* p = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(nodes) +
+                                                      uint32_t(i) * uint32_t(sizeof(DevNode)));
     const float4 a = p[0], b = p[1], c = p[2];
     const int4 m = reinterpret_cast<const int4*>(p)[3];
     DevNode n;
@@ -286,7 +298,7 @@ __device__ __forceinline__ bool traverseBlas(const DevScene& sc, int32_t root, u
             const uint32_t enc = uint32_t(~ref);
             const uint32_t first = enc >> kLeafCountBits, count = enc & ((1u << kLeafCountBits) - 1);
             for (uint32_t i = 0; i < count; ++i) {
-                const float4* tp = reinterpret_cast<const float4*>(sh.tris + first + i);
+                const float4* tp = triOf(sc, sh, first + i);
                 const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
                 float t, v, w, det;
                 if (triHit(sh, v0, v1, v2, r.tmin, r.tmax, t, v, w, det)) {
@@ -500,7 +512,7 @@ struct RayQuery {
             // between lanes at leaves and lanes at nodes.
             const uint32_t n = count < kLeafTrisPerStep ? count : kLeafTrisPerStep;
             for (uint32_t i = 0; i < n; ++i) {
-                const float4* tp = reinterpret_cast<const float4*>(sh.tris + first + i);
+                const float4* tp = triOf(sc, sh, first + i);
                 const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
                 float t, v, w, det;
                 if (triHit(sh, v0, v1, v2, r.tmin, r.tmax, t, v, w, det)) {
