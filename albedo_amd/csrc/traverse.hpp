@@ -410,21 +410,32 @@ __device__ __forceinline__ bool traceRay(const DevScene& sc, const RayF& r, HitF
     }
 }
 
-// The same two-level query as a resumable state machine: step() does one node
-// visit or one leaf, so a lane whose ray has finished can take a new ray while
-// the other lanes of its wave continue (see kernels that refill lanes). Visits
-// the same nodes and primitives with the same culling as traceRay, so results
-// are identical. A TLAS leaf is taken one object at a time; the leaf's other
+// The two-level query as a resumable state machine: step() does a bounded amount
+// of work, so a lane whose ray has finished can take a new ray while the other
+// lanes of its wave continue (see the kernels that refill lanes).
+//
+// A lane's triangle tests overlap its node visits. A BLAS leaf the traversal
+// reaches moves into a leaf slot and the lane goes on to its next stack entry;
+// each step then tests one triangle of the slot's leaf AND visits the lane's
+// next node. A wave whose lanes sit at different points of their traversals
+// runs both kinds of code in most steps anyway; this way a lane busy with a leaf
+// keeps walking nodes in the same steps, so a ray takes about max(node visits,
+// triangle tests) steps rather than their sum (trefoil sets: 3 % more rays per
+// second than one node visit or one triangle per step). A leaf that arrives
+// while the slot is busy waits in `ref`, and an instance is left only once its
+// triangles are done (the triangle test uses the instance's ray). Pushes happen
+// where traceRay pushes, and a pop only ever replaces a leaf that has moved to
+// the slot, so the stack never holds more than traceRay's does.
+//
+// The culling limit lags by the triangles still in the slot, which can add node
+// visits but never loses a hit: the answer is the lexicographic minimum of
+// (t, object, prim) however the nodes are visited, so it is bit for bit
+// traceRay's. A TLAS leaf is taken one object at a time; the leaf's other
 // objects wait as a shorter leaf reference, either on the stack (one entry more
 // than traceRay's) or, with LEAF_IN_REG, in a register (pending) taken up before
 // the next TLAS stack entry (no extra entry, one more live register and check).
 // SPHERES = false compiles the FP64 sphere test out, for scenes whose objects are
 // all mesh instances.
-// Triangles a step tests: with one, a step costs about the same at a leaf as at
-// a node, and lanes of a wave at different kinds of visit waste least (measured
-// on the trefoil sets: 6 % faster than whole leaves of up to 4).
-constexpr uint32_t kLeafTrisPerStep = 1;
-
 template <int BLOCK, bool ANY, bool SPHERES = true, bool LEAF_IN_REG = false>
 struct RayQuery {
     RayF r;          // world-space ray
@@ -432,8 +443,9 @@ struct RayQuery {
     ShearRay sh;     // triangle-test form of the instance-space ray
     HitF best;
     float lim;       // box-culling limit: cullLimit of best.t, updated with it
-    int32_t ref;
-    int32_t pending; // the current TLAS leaf's objects not yet taken, or kNoChild
+    int32_t ref;     // next node or leaf to visit; kNoChild when the lane must pop
+    int32_t leaf;    // BLAS leaf whose triangles are being tested, or kNoChild
+    int32_t pending; // the current TLAS leaf's objects not yet taken (LEAF_IN_REG), or kNoChild
     int sp, sp_blas; // stack depth, and where the current instance's entries start
     uint32_t obj;    // instance being traversed
     bool in_blas, done;
@@ -448,29 +460,42 @@ struct RayQuery {
         lim = cullLimit(r.tmax);
         box = makeBoxRay(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz);
         ref = sc.tlas_root;
+        leaf = kNoChild;
         pending = kNoChild;
         sp = sp_blas = 0;
         in_blas = false;
         done = sc.tlas_root == kNoChild;
     }
 
-    // True when an any-hit query has found its hit.
     __device__ __forceinline__ bool found() const { return ANY && best.object != kMissId; }
 
-    __device__ __forceinline__ void pop(LdsStack<BLOCK> stk)
+    // Moves a BLAS leaf in `ref` into a free slot, and takes the next entry when
+    // `ref` is empty: the rest of the TLAS leaf (LEAF_IN_REG) or the stack top,
+    // leaving the instance first when its entries are used up and its last
+    // triangles tested.
+    __device__ __forceinline__ void absorb()
     {
+        if (in_blas && ref < 0 && ref != kNoChild && leaf == kNoChild) {
+            leaf = ref;
+            ref = kNoChild;
+        }
+    }
+
+    __device__ __forceinline__ void refill(LdsStack<BLOCK> stk)
+    {
+        if (ref != kNoChild) return;
         if (in_blas && sp == sp_blas) {
-            // Back in the TLAS: the world-space box ray again.
+            if (leaf != kNoChild) return;
             in_blas = false;
             box = makeBoxRay(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz);
         }
         if (LEAF_IN_REG && !in_blas && pending != kNoChild) {
-            // The rest of the TLAS leaf comes before the next stack entry.
             ref = pending;
             pending = kNoChild;
             return;
         }
         if (sp == 0) {
+            // Not in an instance here, so the leaf slot is empty too.
             done = true;
             return;
         }
@@ -479,70 +504,62 @@ struct RayQuery {
 
     __device__ __forceinline__ void step(const DevScene& sc, LdsStack<BLOCK> stk)
     {
-        if (ref >= 0) visitNode(sc, stk);
-        else visitLeaf(sc, stk);
-    }
-
-    __device__ __forceinline__ void visitNode(const DevScene& sc, LdsStack<BLOCK> stk)
-    {
-        const DevNode n = loadNode(sc.nodes, ref);
-        bool h0, h1;
-        float t0, t1;
-        testChildren(n, box, r.tmin, lim, h0, h1, t0, t1);
-        if (h0 && h1) {
-            const bool swap = t1 < t0;
-            stk.put(sp++, swap ? n.child[0] : n.child[1]);
-            ref = swap ? n.child[1] : n.child[0];
-            return;
-        }
-        if (h0 || h1) {
-            ref = h0 ? n.child[0] : n.child[1];
-            return;
-        }
-        pop(stk);
-    }
-
-    __device__ __forceinline__ void visitLeaf(const DevScene& sc, LdsStack<BLOCK> stk)
-    {
-        const uint32_t enc = uint32_t(~ref);
-        const uint32_t first = enc >> kLeafCountBits, count = enc & ((1u << kLeafCountBits) - 1);
-        if (in_blas) {
-            // At most kLeafTrisPerStep triangles per step; the rest of the leaf
-            // stays in ref for the next step, so a step's cost varies less
-            // between lanes at leaves and lanes at nodes.
-            const uint32_t n = count < kLeafTrisPerStep ? count : kLeafTrisPerStep;
-            for (uint32_t i = 0; i < n; ++i) {
-                const float4* tp = triOf(sc, sh, first + i);
-                const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
-                float t, v, w, det;
-                if (triHit(sh, v0, v1, v2, r.tmin, r.tmax, t, v, w, det)) {
-                    const uint32_t prim = __float_as_uint(v0.w);
-                    if (ANY || betterHit(t, obj, prim, best)) {
-                        best.t = t;
-                        best.object = obj;
-                        best.prim = prim;
-                        // Any-hit answers only whether something was hit.
-                        best.b1 = ANY ? 0.0f : v / det;
-                        best.b2 = ANY ? 0.0f : w / det;
-                        if (!ANY) lim = cullLimit(t);
-                    }
+        const bool has_leaf = leaf != kNoChild;
+        const bool has_node = ref >= 0;
+        if (has_leaf) {
+            const uint32_t enc = uint32_t(~leaf);
+            const uint32_t first = enc >> kLeafCountBits, count = enc & ((1u << kLeafCountBits) - 1);
+            const float4* tp = triOf(sc, sh, first);
+            const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
+            float t, v, w, det;
+            if (triHit(sh, v0, v1, v2, r.tmin, r.tmax, t, v, w, det)) {
+                const uint32_t prim = __float_as_uint(v0.w);
+                if (ANY || betterHit(t, obj, prim, best)) {
+                    best.t = t;
+                    best.object = obj;
+                    best.prim = prim;
+                    best.b1 = ANY ? 0.0f : v / det;
+                    best.b2 = ANY ? 0.0f : w / det;
+                    if (!ANY) lim = cullLimit(t);
                 }
             }
+            leaf = count > 1 ? encodeLeaf(first + 1, count - 1) : kNoChild;
             if (found()) {
                 done = true;
                 return;
             }
-            if (count > n) {
-                ref = encodeLeaf(first + n, count - n);
-                return;
+        }
+        if (has_node) {
+            const DevNode n = loadNode(sc.nodes, ref);
+            bool h0, h1;
+            float t0, t1;
+            testChildren(n, box, r.tmin, lim, h0, h1, t0, t1);
+            if (h0 && h1) {
+                const bool swap = t1 < t0;
+                stk.put(sp++, swap ? n.child[0] : n.child[1]);
+                ref = swap ? n.child[1] : n.child[0];
+            } else if (h0 || h1) {
+                ref = h0 ? n.child[0] : n.child[1];
+            } else {
+                ref = kNoChild;
             }
-            pop(stk);
-            return;
+        } else if (!in_blas && ref != kNoChild) {
+            takeObject(sc, stk);
+            if (done) return;
         }
-        if (count == 0) {
-            pop(stk);
-            return;
-        }
+        absorb();
+        refill(stk);
+        absorb();
+    }
+
+    // A TLAS leaf in `ref`: its first object is tested (sphere) or entered
+    // (instance); the others wait as a shorter leaf reference.
+    __device__ __forceinline__ void takeObject(const DevScene& sc, LdsStack<BLOCK> stk)
+    {
+        const uint32_t enc = uint32_t(~ref);
+        const uint32_t first = enc >> kLeafCountBits, count = enc & ((1u << kLeafCountBits) - 1);
+        ref = kNoChild;
+        if (count == 0) return;
         const uint32_t o = sc.leafobj[first];
         if (count > 1) {
             if (LEAF_IN_REG) pending = encodeLeaf(first + 1, count - 1);
@@ -562,11 +579,7 @@ struct RayQuery {
                     if (!ANY) lim = cullLimit(tf);
                 }
             }
-            if (found()) {
-                done = true;
-                return;
-            }
-            pop(stk);
+            if (found()) done = true;
             return;
         }
         float m[12];
@@ -577,10 +590,7 @@ struct RayQuery {
         m[8] = m2.x; m[9] = m2.y; m[10] = m2.z; m[11] = m2.w;
         const RayF ro = toObject(m, r);
         sh = makeShear(ro.ox, ro.oy, ro.oz, ro.dx, ro.dy, ro.dz);
-        if (!sh.valid) {
-            pop(stk);
-            return;
-        }
+        if (!sh.valid) return;
         sh.useCopy(sc);
         box = makeBoxRay(ro.ox, ro.oy, ro.oz, ro.dx, ro.dy, ro.dz);
         obj = o;
