@@ -31,6 +31,7 @@ ESTIMATOR_PATH, ESTIMATOR_SCATTER_ONLY, ESTIMATOR_NORMALS = 0, 1, 2
 SCHED_FIFO, SCHED_SHARE, SCHED_SMALL_FIRST = 0, 1, 2
 JOB_KEEP_SUMS = 1
 BUILD_DEVICE_BLAS = 1
+BUILD_WIDE_BVH = 2
 SUMS_FRACTION_BITS = 28
 
 AB_MISS = 0xFFFFFFFF

@@ -126,6 +126,10 @@ ab_status ab_scene_create(ab_context* ctx, const ab_scene_desc* desc, ab_scene**
         (hs.tlas_depth ? hs.tlas_depth - 1 : 0) + (hs.max_blas_depth ? hs.max_blas_depth - 1 : 0);
     if (stack > kMaxStackDepth)
         return fail(AB_INVALID_ARGUMENT, "ab_scene_create: BVH too deep for the traversal stack");
+    // The four-wide form (AB_BUILD_WIDE_BVH) pushes up to three entries a visit.
+    const uint32_t wide_stack = hs.wide_tlas_stack + hs.wide_blas_stack;
+    if (hs.wide_tlas_root != kNoChild && wide_stack > kMaxStackDepth)
+        return fail(AB_INVALID_ARGUMENT, "ab_scene_create: four-wide BVH too deep for the traversal stack");
 
     // One arena holds every array, each 256-byte aligned. Triangles take three
     // copies (DevScene::tris): the built ones go into the third, whose
@@ -210,6 +214,9 @@ ab_status ab_scene_create(ab_context* ctx, const ab_scene_desc* desc, ab_scene**
     s->dev.sky_light = hs.sky_light;
     s->dev.stack_depth = stack;
     s->dev.tri_count = uint32_t(n_tris);
+    // The four-wide form covers host-built trees only.
+    s->dev.wide_root = hs.wide_tlas_root;
+    s->dev.wide_stack = wide_stack;
     s->dev.has_spheres = 0;
     s->dev.has_meshes = 0;
     for (const DevObject& o : hs.objects) {

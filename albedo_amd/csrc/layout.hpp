@@ -62,6 +62,8 @@ struct alignas(16) DevObject {
     uint32_t tri_count;  // kept triangles of the mesh (instances): the bound of a hit's prim id
     float to_object[12]; // row-major 3x4, FP32
     float to_world[12];  // row-major 3x4, FP32
+    int32_t wide_root;   // child reference of the mesh's four-wide BLAS root (instances), or kNoChild
+    uint32_t pad[3];
 };
 
 // Per kept triangle data needed only by shading (not by traversal).
@@ -125,6 +127,11 @@ struct DevScene {
     uint32_t has_spheres; // any object is a sphere (else ray queries skip the FP64 test)
     uint32_t has_meshes;  // any object is a mesh instance (the path tracer then traces with lane refill)
     uint32_t tri_count;   // triangles per copy in tris
+    // The same trees four wide (wide nodes are two consecutive DevNodes, children
+    // 0-1 then 2-3, at an even index): the TLAS root, or kNoChild when the scene
+    // has no four-wide form, and the most stack entries a four-wide query holds.
+    int32_t wide_root;
+    uint32_t wide_stack;
 };
 
 } // namespace ab

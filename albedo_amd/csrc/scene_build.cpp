@@ -251,6 +251,81 @@ int32_t flatten(const SahBuilder& b, int32_t root, uint32_t leaf_base, std::vect
     return ref(root, ref);
 }
 
+// Emits the four-wide form of `tree` under `root` into `out`. A wide node is
+// two consecutive DevNodes holding children 0-1 and 2-3, starting at an even
+// index so it fills one 128-byte line; absent children are kNoChild. A node's
+// children are gathered by opening, while it has fewer than four, the interior
+// child of largest surface area (the usual collapse of a binary SAH tree;
+// leaves stay as built). Returns the root reference; `stack` gets the most
+// entries a traversal holds below this root: a visit pushes all but one of the
+// children it enters, so the bound is the largest sum of (children - 1) over a
+// root-to-node path.
+int32_t flattenWide(const SahBuilder& b, int32_t root, uint32_t leaf_base, std::vector<DevNode>& out,
+                    uint32_t& stack)
+{
+    auto emit = [&](int32_t t, uint32_t& bound, auto&& self) -> int32_t {
+        const TreeNode& tn = b.tree[t];
+        bound = 0;
+        if (tn.left < 0) return encodeLeaf(leaf_base + tn.first, tn.count);
+        int32_t kids[4] = {tn.left, tn.right, -1, -1};
+        int nk = 2;
+        while (nk < 4) {
+            int open = -1;
+            double open_area = -1;
+            for (int i = 0; i < nk; ++i) {
+                const TreeNode& c = b.tree[kids[i]];
+                if (c.left >= 0 && c.box.area() > open_area) {
+                    open = i;
+                    open_area = c.box.area();
+                }
+            }
+            if (open < 0) break;
+            const TreeNode& c = b.tree[kids[open]];
+            kids[open] = c.left;
+            kids[nk++] = c.right;
+        }
+        if (out.size() & 1) out.push_back(DevNode{});
+        const int32_t me = int32_t(out.size());
+        out.resize(out.size() + 2);
+        DevNode half[2]{};
+        uint32_t below = 0;
+        for (int i = 0; i < 4; ++i) {
+            int32_t ref = kNoChild;
+            if (i < nk) {
+                uint32_t cb = 0;
+                ref = self(kids[i], cb, self);
+                below = std::max(below, cb);
+                putBox(half[i >> 1], i & 1, outward(b.tree[kids[i]].box));
+            } else {
+                putBox(half[i >> 1], i & 1, BBox{});
+            }
+            half[i >> 1].child[i & 1] = ref;
+        }
+        out[me] = half[0];
+        out[me + 1] = half[1];
+        bound = below + uint32_t(nk - 1);
+        return me;
+    };
+    const TreeNode& rn = b.tree[root];
+    if (rn.left < 0) {
+        // A lone leaf: a wide node with one child.
+        if (out.size() & 1) out.push_back(DevNode{});
+        const int32_t me = int32_t(out.size());
+        DevNode half[2]{};
+        putBox(half[0], 0, outward(rn.box));
+        putBox(half[0], 1, BBox{});
+        putBox(half[1], 0, BBox{});
+        putBox(half[1], 1, BBox{});
+        half[0].child[0] = encodeLeaf(leaf_base + rn.first, rn.count);
+        half[0].child[1] = half[1].child[0] = half[1].child[1] = kNoChild;
+        out.push_back(half[0]);
+        out.push_back(half[1]);
+        stack = 0;
+        return me;
+    }
+    return emit(root, stack, emit);
+}
+
 bool finite3(const double* p) { return std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]); }
 
 void cross3(const double* a, const double* b, double* o)
@@ -280,6 +355,8 @@ BBox transformBox(const double* m, const BBox& b)
 
 struct MeshBuild {
     int32_t root{kNoChild};
+    int32_t wide_root{kNoChild};
+    uint32_t wide_stack{0};
     uint32_t shading_base{0};
     uint32_t tri_count{0}; // kept triangles
     uint32_t depth{0};
@@ -344,10 +421,15 @@ ab_status buildHostScene(const ab_scene_desc& d, HostScene& out, std::string& er
         err = "scene description has a null array with a non-zero count";
         return AB_INVALID_ARGUMENT;
     }
-    if (d.build_flags & ~AB_BUILD_DEVICE_BLAS) {
+    if (d.build_flags & ~(AB_BUILD_DEVICE_BLAS | AB_BUILD_WIDE_BVH)) {
         err = "unknown scene build flags " + std::to_string(d.build_flags);
         return AB_INVALID_ARGUMENT;
     }
+    if ((d.build_flags & AB_BUILD_DEVICE_BLAS) && (d.build_flags & AB_BUILD_WIDE_BVH)) {
+        err = "AB_BUILD_WIDE_BVH covers host-built trees only; it cannot be combined with AB_BUILD_DEVICE_BLAS";
+        return AB_INVALID_ARGUMENT;
+    }
+    const bool wide = (d.build_flags & AB_BUILD_WIDE_BVH) != 0;
     // ------------------------------------------------------------ materials
     for (uint32_t i = 0; i < d.num_materials; ++i) {
         const ab_material_desc& m = d.materials[i];
@@ -576,6 +658,10 @@ ab_status buildHostScene(const ab_scene_desc& d, HostScene& out, std::string& er
         b.depth = builder.depth_reached + 1;
         out.blas_nodes += b.nodes;
         blas_depth = std::max(blas_depth, b.depth);
+        if (wide) {
+            b.wide_root = flattenWide(builder, root, tri_base, out.nodes, b.wide_stack);
+            out.wide_blas_stack = std::max(out.wide_blas_stack, b.wide_stack);
+        }
     }
     if (out.tris.size() >= (1u << (31 - kLeafCountBits))) {
         err = "scene has too many triangles for the leaf encoding";
@@ -599,6 +685,7 @@ ab_status buildHostScene(const ab_scene_desc& d, HostScene& out, std::string& er
         dobj.material = o.material;
         dobj.light = o.light;
         dobj.blas_root = kNoChild;
+        dobj.wide_root = kNoChild;
         Item it;
         it.id = oi;
         if (o.kind == AB_OBJ_SPHERE) {
@@ -634,6 +721,7 @@ ab_status buildHostScene(const ab_scene_desc& d, HostScene& out, std::string& er
                 return AB_INVALID_ARGUMENT;
             }
             dobj.blas_root = b.root;
+            dobj.wide_root = b.wide_root;
             dobj.mesh = o.mesh;
             dobj.tri_shading_base = b.shading_base;
             dobj.tri_count = b.tri_count;
@@ -709,6 +797,7 @@ ab_status buildHostScene(const ab_scene_desc& d, HostScene& out, std::string& er
         out.tlas_root = flatten(builder, root, leaf_base, out.nodes);
         out.tlas_nodes = out.nodes.size() - before;
         out.tlas_depth = builder.depth_reached + 1;
+        if (wide) out.wide_tlas_root = flattenWide(builder, root, leaf_base, out.nodes, out.wide_tlas_stack);
     }
     out.max_blas_depth = blas_depth;
     out.hash = hashSceneDesc(d);

@@ -30,6 +30,11 @@ struct HostScene {
     std::vector<DevMaterial> materials;
     std::vector<DevLight> lights;
     int32_t tlas_root{kNoChild};
+    // Four-wide form of the same trees (DevScene::wide_root): its TLAS root and
+    // the most stack entries its TLAS and its deepest BLAS need.
+    int32_t wide_tlas_root{kNoChild};
+    uint32_t wide_tlas_stack{0};
+    uint32_t wide_blas_stack{0};
     int32_t sky_light{-1};
     uint32_t tlas_depth{0};
     uint32_t max_blas_depth{0};

@@ -40,12 +40,17 @@ __device__ __forceinline__ RayF loadRay(const float4* rays, uint64_t i)
 
 // Registers the compiler may use: with the FP64 sphere test compiled out the
 // query fits 72 VGPRs (7 waves per SIMD) with few spills; with it, 80 (6 waves).
-template <bool SPHERES>
-constexpr int queryWaves() { return SPHERES ? 6 : 7; }
+// The four-wide visit needs 76 (6 waves).
+template <bool SPHERES, bool WIDE>
+constexpr int queryWaves() { return SPHERES || WIDE ? 6 : 7; }
+
+// A stack class for four-wide trees: 26 entries (6.5 KB per wave) leave LDS
+// room for 6 waves per SIMD; the trefoil's bound is 25.
+constexpr uint32_t kWideStack = 26;
 
 // One kernel for closest and any hit: `per_lane` rays per lane of each wave's slice.
-template <int STACK, bool ANY, bool SPHERES, bool LEAF_IN_REG>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(queryWaves<SPHERES>(), 8))) void kQuery(
+template <int STACK, bool ANY, bool SPHERES, bool LEAF_IN_REG, bool WIDE>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(queryWaves<SPHERES, WIDE>(), 8))) void kQuery(
     DevScene sc, const float4* __restrict__ rays, uint4* __restrict__ hits, uint8_t* __restrict__ occluded, uint64_t n,
     uint32_t per_lane)
 {
@@ -55,7 +60,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(queryWav
     uint64_t next = uint64_t(xcdRemap(blockIdx.x, gridDim.x)) * slice;
     const uint64_t end = next + slice < n ? next + slice : n;
     const LdsStack<kBlock> stk{stack + threadIdx.x};
-    RayQuery<kBlock, ANY, SPHERES, LEAF_IN_REG> q;
+    RayQuery<kBlock, ANY, SPHERES, LEAF_IN_REG, WIDE> q;
     constexpr uint64_t kNone = ~0ull;
     uint64_t id = kNone;
     while (true) {
@@ -86,7 +91,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(queryWav
     }
 }
 
-template <int STACK, bool ANY, bool LEAF_IN_REG>
+template <int STACK, bool ANY, bool LEAF_IN_REG, bool WIDE>
 hipError_t launchQ(const DevScene& sc, const ab_ray* rays, ab_hit* hits, uint8_t* occ, uint64_t n, hipStream_t s)
 {
     const uint64_t per_lane = (n + 64ull * kTargetWaves - 1) / (64ull * kTargetWaves);
@@ -94,21 +99,37 @@ hipError_t launchQ(const DevScene& sc, const ab_ray* rays, ab_hit* hits, uint8_t
     const float4* r = reinterpret_cast<const float4*>(rays);
     uint4* h = reinterpret_cast<uint4*>(hits);
     if (sc.has_spheres)
-        hipLaunchKernelGGL((kQuery<STACK, ANY, true, LEAF_IN_REG>), dim3(uint32_t(waves)), dim3(kBlock), 0, s, sc, r,
-                           h, occ, n, uint32_t(per_lane));
+        hipLaunchKernelGGL((kQuery<STACK, ANY, true, LEAF_IN_REG, WIDE>), dim3(uint32_t(waves)), dim3(kBlock), 0, s,
+                           sc, r, h, occ, n, uint32_t(per_lane));
     else
-        hipLaunchKernelGGL((kQuery<STACK, ANY, false, LEAF_IN_REG>), dim3(uint32_t(waves)), dim3(kBlock), 0, s, sc, r,
-                           h, occ, n, uint32_t(per_lane));
+        hipLaunchKernelGGL((kQuery<STACK, ANY, false, LEAF_IN_REG, WIDE>), dim3(uint32_t(waves)), dim3(kBlock), 0, s,
+                           sc, r, h, occ, n, uint32_t(per_lane));
     return hipGetLastError();
 }
 
 // The query keeps the rest of a TLAS leaf on the stack when that extra entry
 // stays within the scene's stack class (cheaper), else in a register.
-template <int STACK, bool ANY>
-hipError_t launchClass(const DevScene& sc, const ab_ray* rays, ab_hit* hits, uint8_t* occ, uint64_t n, hipStream_t s)
+template <int STACK, bool ANY, bool WIDE>
+hipError_t launchClass(const DevScene& sc, uint32_t depth, const ab_ray* rays, ab_hit* hits, uint8_t* occ,
+                       uint64_t n, hipStream_t s)
 {
-    if (sc.stack_depth + 1 <= uint32_t(STACK)) return launchQ<STACK, ANY, false>(sc, rays, hits, occ, n, s);
-    return launchQ<STACK, ANY, true>(sc, rays, hits, occ, n, s);
+    if (depth + 1 <= uint32_t(STACK)) return launchQ<STACK, ANY, false, WIDE>(sc, rays, hits, occ, n, s);
+    return launchQ<STACK, ANY, true, WIDE>(sc, rays, hits, occ, n, s);
+}
+
+template <bool ANY, bool WIDE>
+hipError_t launchDepth(const DevScene& sc, uint32_t depth, const ab_ray* rays, ab_hit* hits, uint8_t* occ,
+                       uint64_t n, hipStream_t s)
+{
+    if (depth <= kShortStack) return launchClass<kShortStack, ANY, WIDE>(sc, depth, rays, hits, occ, n, s);
+    if constexpr (WIDE) {
+        if (depth + 1 <= kWideStack) return launchClass<kWideStack, ANY, WIDE>(sc, depth, rays, hits, occ, n, s);
+    }
+    if (depth <= 32) return launchClass<32, ANY, WIDE>(sc, depth, rays, hits, occ, n, s);
+    if (depth <= 64) return launchClass<64, ANY, WIDE>(sc, depth, rays, hits, occ, n, s);
+    if (depth <= 128) return launchClass<128, ANY, WIDE>(sc, depth, rays, hits, occ, n, s);
+    if (depth <= kMaxStackDepth) return launchClass<256, ANY, WIDE>(sc, depth, rays, hits, occ, n, s);
+    return hipErrorInvalidValue;
 }
 
 template <bool ANY>
@@ -116,12 +137,8 @@ hipError_t launchByStack(const DevScene& sc, const ab_ray* rays, ab_hit* hits, u
 {
     if (n == 0) return hipSuccess;
     if (n > (uint64_t(1) << 31) * 64) return hipErrorInvalidValue;
-    if (sc.stack_depth <= kShortStack) return launchClass<kShortStack, ANY>(sc, rays, hits, occ, n, s);
-    if (sc.stack_depth <= 32) return launchClass<32, ANY>(sc, rays, hits, occ, n, s);
-    if (sc.stack_depth <= 64) return launchClass<64, ANY>(sc, rays, hits, occ, n, s);
-    if (sc.stack_depth <= 128) return launchClass<128, ANY>(sc, rays, hits, occ, n, s);
-    if (sc.stack_depth <= kMaxStackDepth) return launchClass<256, ANY>(sc, rays, hits, occ, n, s);
-    return hipErrorInvalidValue;
+    if (sc.wide_root != kNoChild) return launchDepth<ANY, true>(sc, sc.wide_stack, rays, hits, occ, n, s);
+    return launchDepth<ANY, false>(sc, sc.stack_depth, rays, hits, occ, n, s);
 }
 
 // Copy c of a triangle: its vertices' coordinates in the order (c + 1, c + 2, c)

@@ -436,7 +436,19 @@ __device__ __forceinline__ bool traceRay(const DevScene& sc, const RayF& r, HitF
 // the next TLAS stack entry (no extra entry, one more live register and check).
 // SPHERES = false compiles the FP64 sphere test out, for scenes whose objects are
 // all mesh instances.
-template <int BLOCK, bool ANY, bool SPHERES = true, bool LEAF_IN_REG = false>
+// Compare-exchange of two (key, child) pairs into ascending key order.
+__device__ __forceinline__ void orderPair(float& ka, int32_t& ca, float& kb, int32_t& cb)
+{
+    const bool swap = kb < ka;
+    const float k = swap ? kb : ka;
+    const int32_t c = swap ? cb : ca;
+    kb = swap ? ka : kb;
+    cb = swap ? ca : cb;
+    ka = k;
+    ca = c;
+}
+
+template <int BLOCK, bool ANY, bool SPHERES = true, bool LEAF_IN_REG = false, bool WIDE = false>
 struct RayQuery {
     RayF r;          // world-space ray
     BoxRay box;      // box-test form of the ray of the current level (world or instance space)
@@ -459,12 +471,12 @@ struct RayQuery {
         best.b1 = best.b2 = 0.0f;
         lim = cullLimit(r.tmax);
         box = makeBoxRay(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz);
-        ref = sc.tlas_root;
+        ref = WIDE ? sc.wide_root : sc.tlas_root;
         leaf = kNoChild;
         pending = kNoChild;
         sp = sp_blas = 0;
         in_blas = false;
-        done = sc.tlas_root == kNoChild;
+        done = ref == kNoChild;
     }
 
     __device__ __forceinline__ bool found() const { return ANY && best.object != kMissId; }
@@ -529,7 +541,31 @@ struct RayQuery {
                 return;
             }
         }
-        if (has_node) {
+        if (has_node && WIDE) {
+            // Four children: the entered ones in ascending entry distance, the
+            // nearest next and the others pushed farthest first.
+            const DevNode n0 = loadNode(sc.nodes, ref), n1 = loadNode(sc.nodes, ref + 1);
+            bool h0, h1, h2, h3;
+            float k0, k1, k2, k3;
+            testChildren(n0, box, r.tmin, lim, h0, h1, k0, k1);
+            testChildren(n1, box, r.tmin, lim, h2, h3, k2, k3);
+            constexpr float kInf = __builtin_huge_valf();
+            int32_t c0 = h0 ? n0.child[0] : kNoChild, c1 = h1 ? n0.child[1] : kNoChild;
+            int32_t c2 = h2 ? n1.child[0] : kNoChild, c3 = h3 ? n1.child[1] : kNoChild;
+            k0 = h0 ? k0 : kInf;
+            k1 = h1 ? k1 : kInf;
+            k2 = c2 != kNoChild ? k2 : kInf;
+            k3 = h3 ? k3 : kInf;
+            orderPair(k0, c0, k1, c1);
+            orderPair(k2, c2, k3, c3);
+            orderPair(k0, c0, k2, c2);
+            orderPair(k1, c1, k3, c3);
+            orderPair(k1, c1, k2, c2);
+            if (c3 != kNoChild) stk.put(sp++, c3);
+            if (c2 != kNoChild) stk.put(sp++, c2);
+            if (c1 != kNoChild) stk.put(sp++, c1);
+            ref = c0;
+        } else if (has_node) {
             const DevNode n = loadNode(sc.nodes, ref);
             bool h0, h1;
             float t0, t1;
@@ -596,7 +632,7 @@ struct RayQuery {
         obj = o;
         in_blas = true;
         sp_blas = sp;
-        ref = d.blas_root;
+        ref = WIDE ? d.wide_root : d.blas_root;
     }
 };
 

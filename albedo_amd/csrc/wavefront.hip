@@ -1119,6 +1119,8 @@ __device__ __forceinline__ DevScene sceneOfTag(const PathState& ps, uint32_t tag
     u.has_spheres = waveUniform(s.has_spheres);
     u.has_meshes = waveUniform(s.has_meshes);
     u.tri_count = waveUniform(s.tri_count);
+    u.wide_root = int32_t(waveUniform(uint32_t(s.wide_root)));
+    u.wide_stack = waveUniform(s.wide_stack);
     return u;
 }
 

@@ -207,9 +207,10 @@ def batch_leg(ctx, n_jobs=16, full=False, forms=("plain", "batched"), tune=True)
 
 def build_leg(ctx):
     """BVH build of the serving scene (1,310,720-triangle displaced icosphere x 64
-    instances): scene creation with the host SAH BLAS vs the device linear BVH
-    (AB_BUILD_DEVICE_BLAS), and closest-hit throughput of each on 2^22 rays into
-    the instance grid (hits are identical; the tree quality differs)."""
+    instances): scene creation with the host SAH BLAS, the device linear BVH
+    (AB_BUILD_DEVICE_BLAS) and the host SAH trees collapsed four wide
+    (AB_BUILD_WIDE_BVH), and closest-hit throughput of each on 2^22 rays into the
+    instance grid (hits are identical; the trees differ)."""
     import torch
     from albedo_amd import workloads as W
     s = W.icosphere_scene()
@@ -228,9 +229,9 @@ def build_leg(ctx):
     d_hits = torch.zeros((n, 4), dtype=torch.int32, device="cuda")
     stream = torch.cuda.current_stream()
     digests = {}
-    for name, device in (("host_sah", False), ("device_lbvh", True)):
+    for name, device, wide in (("host_sah", False, False), ("device_lbvh", True, False), ("host_sah_wide", False, True)):
         t0 = time.perf_counter()
-        g = ctx.scene(s, device_blas=device)
+        g = ctx.scene(s, device_blas=device, wide_bvh=wide)
         build_s = time.perf_counter() - t0
         g.trace_closest_device(d_rays.data_ptr(), d_hits.data_ptr(), n, stream.cuda_stream)
         torch.cuda.synchronize()
@@ -248,7 +249,7 @@ def build_leg(ctx):
                      "stack_depth": info["stack_depth"], "mrays_s": round(n / (min(ms) * 1e-3) / 1e6, 1)}
         g.release()
         print("build: %s done" % name, file=sys.stderr, flush=True)
-    out["hits_identical"] = digests["host_sah"] == digests["device_lbvh"]
+    out["hits_identical"] = digests["host_sah"] == digests["device_lbvh"] == digests["host_sah_wide"]
     return out
 
 

@@ -175,6 +175,17 @@ typedef struct ab_scene_desc {
  * the host). */
 #define AB_BUILD_DEVICE_BLAS 1u
 
+/* Also build a four-wide form of the host-built trees (each node holds four
+ * children: the binary SAH tree collapsed by opening the child of largest
+ * surface area) and trace ray queries (ab_trace_closest, ab_trace_any) over it.
+ * Results are the same bit for bit. On MI355X it is slower than the binary
+ * tree, whose node visit tests two boxes per 64-byte fetch: the queries are
+ * bound by instruction issue, a four-wide visit costs more than two binary ones
+ * and its stack (up to three entries a visit) takes more LDS. Rejected together
+ * with AB_BUILD_DEVICE_BLAS. No counterpart in the reference (accel/bvh.cpp
+ * builds a binary tree). */
+#define AB_BUILD_WIDE_BVH 2u
+
 /* Pinhole camera (reference CameraSettings + PerspectiveCamera,
  * camera/perspective_camera.hpp, perspective_camera.cpp:74-110). */
 typedef struct ab_camera {
