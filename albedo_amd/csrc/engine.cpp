@@ -192,7 +192,11 @@ ab_status prepareJob(const JobSpec& spec, DevJob& dj)
 
 Engine::~Engine() { freeAll(); }
 
-void Engine::clearTags() { tags_.clear(); }
+void Engine::clearTags()
+{
+    tags_.clear();
+    untagged_meshes_ = false;
+}
 
 ab_status Engine::tagFor(const DevScene& scene, uint32_t& tag)
 {
@@ -563,9 +567,21 @@ ab_status Engine::beginStream(const ab_launch_config& cfg, uint32_t max_jobs)
 
 ab_status Engine::traceScenes(const SceneSet* live)
 {
-    LaunchShape next = shapeFor(stream_cfg_, capacity_, shape_.stack);
+    // With the scenes in flight known, the one-ray-per-lane kernels take the stack
+    // class of the scenes they trace: all of them when none has meshes, else the
+    // untagged ones (sphere-only scenes, and every scene once a mesh scene went
+    // without a tag). A deep mesh scene in the mix then no longer costs the sphere
+    // scenes' rays LDS and occupancy.
+    uint32_t stack = shape_.stack;
+    if (live) {
+        const uint32_t depth = !live->meshes || untagged_meshes_
+                                   ? std::max(live->deepest_plain, live->deepest_meshes)
+                                   : live->deepest_plain;
+        stack = stackClass(depth);
+    }
+    LaunchShape next = shapeFor(stream_cfg_, capacity_, stack);
     chooseTracing(next, live ? *live : table_set_, capacity_, live != nullptr, stream_cfg_);
-    if (sameTracing(next, shape_)) return AB_OK;
+    if (next.stack == shape_.stack && sameTracing(next, shape_)) return AB_OK;
     shape_ = next;
     return capture(shape_);
 }
@@ -606,6 +622,7 @@ ab_status Engine::appendJob(const DevJob& prepared, unsigned long long* accum, u
     // paths' queue entries name their scene.
     const ab_status tst = tagFor(dj.scene, dj.tag);
     if (tst != AB_OK) return tst;
+    if (dj.scene.has_meshes && dj.tag == 0) untagged_meshes_ = true;
     const uint32_t idx = uint32_t(jobs_.size());
     jobs_.push_back(dj);
     HIP_TRY(hipMemcpyAsync(d_jobs_ + idx, &jobs_[idx], sizeof(DevJob), hipMemcpyHostToDevice, stream_),

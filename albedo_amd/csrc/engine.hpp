@@ -45,12 +45,14 @@ struct SceneSet {
     bool meshes{false}, same{true}, empty{true};
     bool mesh_spheres{false};  // a scene with meshes also has spheres
     uint32_t deepest_meshes{0}; // largest stack bound of the scenes with meshes
+    uint32_t deepest_plain{0};  // ... of the scenes without
     DevScene first{};
     void add(const DevScene& s)
     {
         if (empty) first = s;
         else if (std::memcmp(&s, &first, sizeof(DevScene)) != 0) same = false;
         empty = false;
+        if (!s.has_meshes) deepest_plain = deepest_plain > s.stack_depth ? deepest_plain : s.stack_depth;
         if (s.has_meshes) {
             meshes = true;
             mesh_spheres = mesh_spheres || s.has_spheres;
@@ -139,6 +141,7 @@ private:
     unsigned long long work_total_{0};
     LaunchShape shape_{};          // serving: current bounce shape
     SceneSet table_set_{};         // serving: the scenes of the job table
+    bool untagged_meshes_{false};  // serving: a job of a scene with meshes got no tag (tags ran out)
     ab_launch_config stream_cfg_{}; // serving: configuration the stream started with
     hipGraphExec_t graph_{nullptr};
     LaunchShape graph_shape_{};     // shape graph_ was captured with

@@ -26,6 +26,9 @@ namespace ab {
 
 namespace {
 
+// Bounces the serving loop runs between two reads of the device's counters.
+constexpr unsigned long long kRoundBounces = 8;
+
 enum JobState : uint32_t { kQueued = 0, kRunning = 1, kDone = 2, kCancelled = 3, kFailed = 4 };
 
 struct ServerJob {
@@ -98,7 +101,6 @@ struct Server {
     std::vector<double> latencies;
     hipStream_t poll_stream{nullptr};
     Counters last{};                    // engine counters at the last read (stream idle)
-    unsigned long long last_consumed{0}; // samples generated between the last two reads
     size_t rr{0};                       // SHARE: rotates which running job is served first
     uint32_t policy{AB_SCHED_SMALL_FIRST}; // guarded by mu
     unsigned long long* h_done{nullptr}; // pinned: finished-sample lanes of the running jobs
@@ -156,10 +158,16 @@ ab_status Server::admitLocked()
 
 // The engine generates work in the order it is exposed, so exposing each job's
 // samples whole on admission would make a job that arrives behind a large one
-// wait for every sample of it. Instead about one window of samples is kept
-// exposed ahead of generation (twice the larger of the path capacity and what
-// the last round of bounces generated, so the device never runs dry within a
-// round) and each round's refill is handed out by the policy:
+// wait for every sample of it. Instead one window of samples is kept exposed
+// ahead of generation and each round's refill is handed out by the policy. The
+// window is (kRoundBounces + 1) path capacities: a bounce generates at most one
+// capacity of samples, so every bounce of the round still finds at least a full
+// queue's worth exposed and new samples take the terminated paths' slots and
+// queue positions. With less, the queue is compacted whenever it runs short in
+// a round; its slot order then stays scattered for the rest of the job, and
+// shading read 38 % more bytes for the same work (a window of two capacities:
+// the 24-job serving mix one job at a time through the server took 8.35 s, the
+// plain path 7.74 s; with this window 7.86 s). Policies:
 //   FIFO: in admission order;
 //   SHARE: split evenly between the jobs with samples left, rotating which
 //          job goes first (processor sharing);
@@ -172,7 +180,7 @@ ab_status Server::scheduleLocked()
     constexpr unsigned long long kMinSlice = 1ull << 16;
     const unsigned long long cap = engine.capacity();
     const unsigned long long ahead = engine.workTotal() - last.work_next;
-    const unsigned long long window = std::max<unsigned long long>(2 * cap, 2 * last_consumed);
+    const unsigned long long window = (kRoundBounces + 1) * cap;
     if (ahead < window && !running.empty()) {
         std::vector<ServerJob*> want; // running jobs with samples left, admission order
         for (uint64_t id : running) {
@@ -319,10 +327,15 @@ void Server::loop()
                 thread_status = sst;
                 break;
             }
-            // Paths in flight belong to the running jobs (a job retires only once
-            // all its samples have finished) and to the draining cancelled ones.
+            // Paths in flight, or generated in the coming round, belong to the
+            // running jobs with exposed samples not known to have finished (`done`
+            // lags the device, so this may hold more than needed, never less) and
+            // to the draining cancelled ones. A job admitted but not yet exposed has
+            // none, so it does not force several-scene tracing while the device
+            // still works on others.
             live_set = SceneSet{};
-            for (uint64_t id : running) live_set.add(jobs[id]->prepared.scene);
+            for (uint64_t id : running)
+                if (jobs[id]->exposed > jobs[id]->done) live_set.add(jobs[id]->prepared.scene);
             for (const auto& j : draining) live_set.add(j->prepared.scene);
         }
         if (engine.traceScenes(&live_set) != AB_OK) {
@@ -332,13 +345,12 @@ void Server::loop()
         }
         const double t0 = msSince(ctx);
         Counters c{};
-        if (engine.runBounces(8) != AB_OK || engine.readCounters(c) != AB_OK) {
+        if (engine.runBounces(kRoundBounces) != AB_OK || engine.readCounters(c) != AB_OK) {
             std::lock_guard<std::mutex> lock(mu);
             thread_status = AB_DEVICE_ERROR;
             break;
         }
         busy_ms += msSince(ctx) - t0;
-        last_consumed = c.work_next - last.work_next;
         last = c;
         rays_now = c.extend_rays + c.shadow_rays;
         // A cancelled job's paths end at their first shading after the flag: once
