@@ -225,6 +225,10 @@ void Engine::freeAll()
     if (arena_) (void)hipFree(arena_);
     if (accum_) (void)hipFree(accum_);
     if (h_counters_) (void)hipHostFree(h_counters_);
+    for (hipEvent_t& e : counter_ev_) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
     if (h_segs_) (void)hipHostFree(h_segs_);
     if (stream_) (void)hipStreamDestroy(stream_);
     arena_ = nullptr;
@@ -244,7 +248,7 @@ ab_status Engine::reserve(uint32_t capacity, uint32_t max_jobs, hipStream_t)
     freeAll();
     HIP_TRY(hipSetDevice(ctx_->device), "hipSetDevice");
     HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
-    HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&h_counters_), sizeof(Counters), hipHostMallocDefault),
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&h_counters_), 3 * sizeof(Counters), hipHostMallocDefault),
             "hipHostMalloc(counters)");
     uint32_t seg_cap = 4096;
     while (seg_cap < max_jobs) seg_cap *= 2;
@@ -306,18 +310,46 @@ ab_status Engine::bouncePair(const LaunchShape& shape)
     return AB_OK;
 }
 
+namespace {
+
+// Counters as the host uses them: the last bounce's terminations and shadow
+// rays, which the device adds at the next bounce, counted now.
+Counters settled(Counters c)
+{
+    c.paths_done += c.n_term;
+    c.shadow_rays += c.n_shadow;
+    c.n_free += c.n_term - c.n_recycle;
+    c.n_term = c.n_shadow = 0;
+    return c;
+}
+
+} // namespace
+
 ab_status Engine::readCounters(Counters& out)
 {
     HIP_TRY(hipMemcpyAsync(h_counters_, ps_.ctr, sizeof(Counters), hipMemcpyDeviceToHost, stream_),
             "hipMemcpy(counters)");
     HIP_TRY(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-    out = *h_counters_;
-    // The last bounce's terminations and shadow rays, which the device adds at
-    // the next bounce, counted now.
-    out.paths_done += out.n_term;
-    out.shadow_rays += out.n_shadow;
-    out.n_free += out.n_term - out.n_recycle;
-    out.n_term = out.n_shadow = 0;
+    out = settled(*h_counters_);
+    return AB_OK;
+}
+
+ab_status Engine::enqueueCounters(uint32_t slot)
+{
+    if (slot > 1) return fail(AB_INVALID_ARGUMENT, "counter slot must be 0 or 1");
+    if (!counter_ev_[slot])
+        HIP_TRY(hipEventCreateWithFlags(&counter_ev_[slot], hipEventDisableTiming), "hipEventCreate(counters)");
+    HIP_TRY(hipMemcpyAsync(h_counters_ + 1 + slot, ps_.ctr, sizeof(Counters), hipMemcpyDeviceToHost, stream_),
+            "hipMemcpy(counters)");
+    HIP_TRY(hipEventRecord(counter_ev_[slot], stream_), "hipEventRecord(counters)");
+    return AB_OK;
+}
+
+ab_status Engine::waitCounters(uint32_t slot, Counters& out)
+{
+    if (slot > 1 || !counter_ev_[slot]) return fail(AB_INVALID_ARGUMENT, "no counter read in that slot");
+    HIP_TRY(hipEventSynchronize(counter_ev_[slot]), "hipEventSynchronize(counters)");
+    out = settled(h_counters_[1 + slot]);
     return AB_OK;
 }
 
@@ -330,6 +362,9 @@ ab_status Engine::capture(const LaunchShape& shape)
         shape.stack == graph_shape_.stack && shape.pstack == graph_shape_.pstack && sameTracing(shape, graph_shape_))
         return AB_OK;
     if (graph_) {
+        // The serving loop enqueues a round before it reads the last one, so the
+        // graph may still run: let it finish before it is replaced.
+        HIP_TRY(hipStreamSynchronize(stream_), "hipStreamSynchronize(graph)");
         (void)hipGraphExecDestroy(graph_);
         graph_ = nullptr;
     }

@@ -103,6 +103,12 @@ public:
     bool tableFull() const { return jobs_.size() >= max_jobs_; }
     size_t tableSize() const { return jobs_.size(); }
     ab_status readCounters(Counters& out);
+    // The same read without waiting: a copy into pinned slot `slot` (0 or 1) and
+    // an event after it, enqueued behind the work so far; waitCounters waits for
+    // that event and returns the slot's counters. The serving loop reads one
+    // round's counters while the next round runs.
+    ab_status enqueueCounters(uint32_t slot);
+    ab_status waitCounters(uint32_t slot, Counters& out);
     bool idle(const Counters& c) const { return c.work_next == c.work_total && c.n_active == 0; }
     uint32_t capacity() const { return capacity_; }
     hipStream_t stream() const { return stream_; }
@@ -129,7 +135,8 @@ private:
     uint32_t max_jobs_{0};
     PathState ps_{};
     DevJob* d_jobs_{nullptr};
-    Counters* h_counters_{nullptr}; // pinned
+    Counters* h_counters_{nullptr}; // pinned: readCounters' copy, then the two enqueueCounters slots
+    hipEvent_t counter_ev_[2]{nullptr, nullptr};
     DevSegment* d_segs_{nullptr};   // segment ring on the device
     DevSegment* h_segs_{nullptr};   // its host mirror (pinned); copies read it when they run
     uint32_t seg_cap_{0};           // ring entries, a power of two

@@ -26,8 +26,10 @@ namespace ab {
 
 namespace {
 
-// Bounces the serving loop runs between two reads of the device's counters.
-constexpr unsigned long long kRoundBounces = 8;
+// Bounces of one round of the serving loop. Rounds are pipelined: the loop
+// enqueues round k + 1 before it reads round k's counters, so the host retires,
+// admits and schedules while the device runs.
+constexpr unsigned long long kRoundBounces = 4;
 
 enum JobState : uint32_t { kQueued = 0, kRunning = 1, kDone = 2, kCancelled = 3, kFailed = 4 };
 
@@ -103,9 +105,21 @@ struct Server {
     Counters last{};                    // engine counters at the last read (stream idle)
     size_t rr{0};                       // SHARE: rotates which running job is served first
     uint32_t policy{AB_SCHED_SMALL_FIRST}; // guarded by mu
-    unsigned long long* h_done{nullptr}; // pinned: finished-sample lanes of the running jobs
+    // Finished-sample counts of the jobs running at the last round, summed on
+    // the device (launchSumDone) into d_done and copied into h_done (pinned)
+    // before the round's counter read, so a round waits for the device once.
+    // Two sets, one per round in flight (the counter slot of the round).
+    std::vector<std::shared_ptr<ServerJob>> round_jobs[2];
+    unsigned long long* d_done[2]{nullptr, nullptr};
+    unsigned long long* h_done[2]{nullptr, nullptr};
+    size_t done_room[2]{0, 0};          // entries of d_done[s] and h_done[s]
+    hipStream_t retire_stream{nullptr}; // reads finished jobs' images while the next round runs
+    // Buffers of jobs retired while a round was in flight, freed once the round
+    // in slot s has been waited for: that round's finished-sample sums read
+    // every job running when it was enqueued. Their bytes stay charged to the
+    // budget until then.
+    std::vector<std::pair<void*, size_t>> to_free[2];
     SceneSet live_set{};                // scenes of the running and draining jobs (loop thread)
-    size_t h_done_jobs{0};              // jobs h_done has room for
 
     Server(ab_context* c, uint64_t b) : ctx(c), budget(b), engine(c) {}
 
@@ -113,7 +127,10 @@ struct Server {
     void loop();
     ab_status admitLocked();
     ab_status scheduleLocked();
-    void retireFinished();
+    ab_status enqueueDoneReads(uint32_t slot);
+    ab_status finishRound(uint32_t slot, int next_slot);
+    void retireFinished(uint32_t slot, int next_slot);
+    void releaseBuffer(ServerJob& j, int next_slot);
 };
 
 ab_status Server::admitLocked()
@@ -139,6 +156,7 @@ ab_status Server::admitLocked()
         uint32_t index = 0;
         const ab_status st = engine.appendJob(j.prepared, acc, done, &index);
         if (st != AB_OK) {
+            (void)hipStreamSynchronize(engine.stream()); // the clearing memset may still be queued
             (void)hipFree(dev);
             j.state = kFailed;
             pending.pop_front();
@@ -160,14 +178,16 @@ ab_status Server::admitLocked()
 // samples whole on admission would make a job that arrives behind a large one
 // wait for every sample of it. Instead one window of samples is kept exposed
 // ahead of generation and each round's refill is handed out by the policy. The
-// window is (kRoundBounces + 1) path capacities: a bounce generates at most one
-// capacity of samples, so every bounce of the round still finds at least a full
+// window is (2 kRoundBounces + 1) path capacities past the generation point of
+// the last counters read: a bounce generates at most one capacity of samples,
+// and the round in flight and the round about to be enqueued both generate
+// before the next exposure, so every bounce still finds at least a full
 // queue's worth exposed and new samples take the terminated paths' slots and
 // queue positions. With less, the queue is compacted whenever it runs short in
 // a round; its slot order then stays scattered for the rest of the job, and
 // shading read 38 % more bytes for the same work (a window of two capacities:
 // the 24-job serving mix one job at a time through the server took 8.35 s, the
-// plain path 7.74 s; with this window 7.86 s). Policies:
+// plain path 7.74 s; with a full round's window 7.86 s). Policies:
 //   FIFO: in admission order;
 //   SHARE: split evenly between the jobs with samples left, rotating which
 //          job goes first (processor sharing);
@@ -180,7 +200,7 @@ ab_status Server::scheduleLocked()
     constexpr unsigned long long kMinSlice = 1ull << 16;
     const unsigned long long cap = engine.capacity();
     const unsigned long long ahead = engine.workTotal() - last.work_next;
-    const unsigned long long window = (kRoundBounces + 1) * cap;
+    const unsigned long long window = (2 * kRoundBounces + 1) * cap;
     if (ahead < window && !running.empty()) {
         std::vector<ServerJob*> want; // running jobs with samples left, admission order
         for (uint64_t id : running) {
@@ -224,40 +244,76 @@ ab_status Server::scheduleLocked()
     return engine.publishWork(last.work_next);
 }
 
-void Server::retireFinished()
+// Enqueues, after the round's bounces, the sums of the running jobs' finished-
+// sample counters and their copy to the host; the counter read that follows
+// waits for them.
+ab_status Server::enqueueDoneReads(uint32_t slot)
 {
-    // Finished-sample counters of the running jobs, read outside the lock (only
-    // this thread frees job buffers): one copy each into pinned memory, one wait.
-    std::vector<std::shared_ptr<ServerJob>> live;
+    std::vector<std::shared_ptr<ServerJob>>& rj = round_jobs[slot];
+    rj.clear();
     {
         std::lock_guard<std::mutex> lock(mu);
-        for (uint64_t id : running) live.push_back(jobs[id]);
+        for (uint64_t id : running) rj.push_back(jobs[id]);
     }
-    if (live.empty()) return;
-    constexpr size_t kLanesPerJob = kDoneBytes / 8;
-    if (live.size() > h_done_jobs) {
-        const size_t n = std::max<size_t>(live.size(), 2 * h_done_jobs + 16);
-        if (h_done) (void)hipHostFree(h_done);
-        h_done = nullptr;
-        h_done_jobs = 0;
-        if (hipHostMalloc(reinterpret_cast<void**>(&h_done), n * kDoneBytes, hipHostMallocDefault) != hipSuccess)
-            return; // checked again next round
-        h_done_jobs = n;
+    if (rj.empty()) return AB_OK;
+    if (rj.size() > done_room[slot]) {
+        // The slot's previous round has been processed, so its buffers are idle.
+        const size_t n = std::max<size_t>(rj.size(), 2 * done_room[slot] + 64);
+        if (d_done[slot]) (void)hipFree(d_done[slot]);
+        if (h_done[slot]) (void)hipHostFree(h_done[slot]);
+        d_done[slot] = h_done[slot] = nullptr;
+        done_room[slot] = 0;
+        if (hipMalloc(reinterpret_cast<void**>(&d_done[slot]), n * sizeof(unsigned long long)) != hipSuccess ||
+            hipHostMalloc(reinterpret_cast<void**>(&h_done[slot]), n * sizeof(unsigned long long),
+                          hipHostMallocDefault) != hipSuccess)
+            return fail(AB_OUT_OF_MEMORY, "server: finished-sample buffers");
+        done_room[slot] = n;
     }
-    for (size_t i = 0; i < live.size(); ++i)
-        if (hipMemcpyAsync(h_done + i * kLanesPerJob, static_cast<char*>(live[i]->dev) + live[i]->buf.done_offset,
-                           kDoneBytes, hipMemcpyDeviceToHost, engine.stream()) != hipSuccess)
-            return;
-    if (hipStreamSynchronize(engine.stream()) != hipSuccess) return;
+    std::vector<unsigned long long*> lanes(rj.size());
+    for (size_t i = 0; i < rj.size(); ++i)
+        lanes[i] = reinterpret_cast<unsigned long long*>(static_cast<char*>(rj[i]->dev) + rj[i]->buf.done_offset);
+    for (size_t i = 0; i < lanes.size(); i += kDoneListMax) {
+        const uint32_t n = uint32_t(std::min<size_t>(kDoneListMax, lanes.size() - i));
+        if (launchSumDone(lanes.data() + i, n, d_done[slot] + i, engine.stream()) != hipSuccess)
+            return fail(AB_DEVICE_ERROR, "server: finished-sample sums");
+    }
+    if (hipMemcpyAsync(h_done[slot], d_done[slot], rj.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                       engine.stream()) != hipSuccess)
+        return fail(AB_DEVICE_ERROR, "server: finished-sample copy");
+    return AB_OK;
+}
+
+// The jobs of the last round, with their finished-sample counts already on the
+// host (the counter read synchronised the stream after enqueueDoneReads):
+// cancellations take effect, and jobs whose samples have all landed retire.
+// Frees a retired job's buffer now, or, when a round is in flight (in
+// `next_slot`), after that round has been waited for. Called with `mu` held.
+void Server::releaseBuffer(ServerJob& j, int next_slot)
+{
+    if (next_slot >= 0) {
+        to_free[next_slot].push_back({j.dev, j.footprint});
+    } else {
+        (void)hipFree(j.dev);
+        used -= j.footprint;
+    }
+    j.dev = nullptr;
+}
+
+void Server::retireFinished(uint32_t slot, int next_slot)
+{
+    // Read outside the lock: only this thread frees job buffers. The round's
+    // jobs are those running when it was enqueued; the previous round's
+    // processing may since have retired or cancelled some of them, which are
+    // then left alone.
+    const std::vector<std::shared_ptr<ServerJob>>& live = round_jobs[slot];
     for (size_t i = 0; i < live.size(); ++i) {
         ServerJob& j = *live[i];
         const size_t npx = size_t(j.prepared.width) * j.prepared.height;
-        const unsigned long long* lanes = h_done + i * kLanesPerJob;
-        unsigned long long d = 0;
-        for (uint32_t k = 0; k < kDoneLanes; ++k) d += lanes[size_t(k) * kDoneStride];
+        const unsigned long long d = h_done[slot][i];
         bool cancel = false;
         {
             std::lock_guard<std::mutex> lock(mu);
+            if (j.state != kRunning) continue;
             j.done = d;
             cancel = j.cancel_requested;
         }
@@ -279,15 +335,16 @@ void Server::retireFinished()
             continue;
         }
         if (d < j.total) continue;
-        // The engine stream is idle here (readCounters synchronised it), so every
-        // sample of the job has landed. The lock keeps polls off the job's mean
-        // buffer meanwhile.
+        // Every sample of the job has landed by the end of the round read (its
+        // counters were waited for), and no later round has paths of the job, so
+        // its image is read on a stream of its own while the next round runs.
+        // The lock keeps polls off the job's mean buffer meanwhile.
         std::lock_guard<std::mutex> lock(mu);
         std::vector<float> result(npx * 3);
-        hipError_t e = readMean(j, result.data(), engine.stream());
+        hipError_t e = readMean(j, result.data(), retire_stream);
         if (e == hipSuccess && (j.flags & AB_JOB_KEEP_SUMS)) {
             j.sums.resize(npx * 4);
-            e = readSums(j, j.sums.data(), engine.stream());
+            e = readSums(j, j.sums.data(), retire_stream);
         }
         if (e == hipSuccess) {
             j.result = std::move(result);
@@ -298,18 +355,68 @@ void Server::retireFinished()
         j.finish_ms = msSince(ctx);
         latencies.push_back(j.finish_ms - j.submit_ms);
         ++jobs_done;
-        (void)hipFree(j.dev);
-        j.dev = nullptr;
-        used -= j.footprint;
+        releaseBuffer(j, next_slot);
         running.erase(std::find(running.begin(), running.end(), j.id));
         cv.notify_all();
     }
 }
 
+// Processes a round whose reads were enqueued in `slot`: waits for them, then
+// drains cancelled jobs, retires finished ones and, when the device has run
+// out of work with no job running and no later round enqueued, resets the job
+// table.
+ab_status Server::finishRound(uint32_t slot, int next_slot)
+{
+    Counters c{};
+    const ab_status st = engine.waitCounters(slot, c);
+    if (st != AB_OK) return st;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        for (const auto& b : to_free[slot]) {
+            (void)hipFree(b.first);
+            used -= b.second;
+        }
+        to_free[slot].clear();
+    }
+    last = c;
+    rays_now = c.extend_rays + c.shadow_rays;
+    // A cancelled job's paths end at their first shading after the flag: once
+    // generation has passed the job's last exposed sample at one counter read,
+    // every path of it was generated and shaded within the bounces before the
+    // next read. Two reads past it, it has no paths left.
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        draining.erase(std::remove_if(draining.begin(), draining.end(),
+                                      [&](const std::shared_ptr<ServerJob>& j) {
+                                          if (c.work_next >= j->work_end) ++j->drain_reads;
+                                          return j->drain_reads >= 2 || engine.idle(c);
+                                      }),
+                       draining.end());
+    }
+    retireFinished(slot, next_slot);
+    if (engine.idle(c) && next_slot < 0) {
+        std::lock_guard<std::mutex> lock(mu);
+        if (running.empty() && engine.tableSize() > 0) {
+            rays_before += rays_now;
+            rays_now = 0;
+            (void)engine.resetTable();
+            last = Counters{};
+        }
+    }
+    return AB_OK;
+}
+
 void Server::loop()
 {
     (void)hipSetDevice(ctx->device);
+    if (hipStreamCreateWithFlags(&retire_stream, hipStreamNonBlocking) != hipSuccess) {
+        std::lock_guard<std::mutex> lock(mu);
+        thread_status = AB_DEVICE_ERROR;
+        return;
+    }
+    int in_flight = -1; // counter slot of the round enqueued and not yet processed
     while (true) {
+        bool enqueue = false;
         {
             std::unique_lock<std::mutex> lock(mu);
             if (stop) break;
@@ -318,66 +425,51 @@ void Server::loop()
                 thread_status = st;
                 break;
             }
-            if (running.empty()) {
+            if (running.empty() && in_flight < 0) {
                 cv.wait_for(lock, std::chrono::milliseconds(2), [&] { return stop || !pending.empty(); });
                 continue;
             }
-            const ab_status sst = scheduleLocked();
-            if (sst != AB_OK) {
-                thread_status = sst;
+            if (!running.empty()) {
+                const ab_status sst = scheduleLocked();
+                if (sst != AB_OK) {
+                    thread_status = sst;
+                    break;
+                }
+                // Paths in flight, or generated in the coming rounds, belong to the
+                // running jobs with exposed samples not known to have finished
+                // (`done` lags the device, so this may hold more than needed, never
+                // less) and to the draining cancelled ones. A job admitted but not
+                // yet exposed has none, so it does not force several-scene tracing
+                // while the device still works on others.
+                live_set = SceneSet{};
+                for (uint64_t id : running)
+                    if (jobs[id]->exposed > jobs[id]->done) live_set.add(jobs[id]->prepared.scene);
+                for (const auto& j : draining) live_set.add(j->prepared.scene);
+                enqueue = true;
+            }
+        }
+        int cur = -1;
+        if (enqueue) {
+            cur = in_flight == 0 ? 1 : 0;
+            const double t0 = msSince(ctx);
+            if (engine.traceScenes(&live_set) != AB_OK || engine.runBounces(kRoundBounces) != AB_OK ||
+                enqueueDoneReads(uint32_t(cur)) != AB_OK || engine.enqueueCounters(uint32_t(cur)) != AB_OK) {
+                std::lock_guard<std::mutex> lock(mu);
+                thread_status = AB_DEVICE_ERROR;
                 break;
             }
-            // Paths in flight, or generated in the coming round, belong to the
-            // running jobs with exposed samples not known to have finished (`done`
-            // lags the device, so this may hold more than needed, never less) and
-            // to the draining cancelled ones. A job admitted but not yet exposed has
-            // none, so it does not force several-scene tracing while the device
-            // still works on others.
-            live_set = SceneSet{};
-            for (uint64_t id : running)
-                if (jobs[id]->exposed > jobs[id]->done) live_set.add(jobs[id]->prepared.scene);
-            for (const auto& j : draining) live_set.add(j->prepared.scene);
+            busy_ms += msSince(ctx) - t0;
         }
-        if (engine.traceScenes(&live_set) != AB_OK) {
-            std::lock_guard<std::mutex> lock(mu);
-            thread_status = AB_DEVICE_ERROR;
-            break;
-        }
-        const double t0 = msSince(ctx);
-        Counters c{};
-        if (engine.runBounces(kRoundBounces) != AB_OK || engine.readCounters(c) != AB_OK) {
-            std::lock_guard<std::mutex> lock(mu);
-            thread_status = AB_DEVICE_ERROR;
-            break;
-        }
-        busy_ms += msSince(ctx) - t0;
-        last = c;
-        rays_now = c.extend_rays + c.shadow_rays;
-        // A cancelled job's paths end at their first shading after the flag: once
-        // generation has passed the job's last exposed sample at one counter read,
-        // every path of it was generated and shaded within the bounces before the
-        // next read. Two reads past it, it has no paths left.
-        {
-            std::lock_guard<std::mutex> lock(mu);
-            draining.erase(std::remove_if(draining.begin(), draining.end(),
-                                          [&](const std::shared_ptr<ServerJob>& j) {
-                                              if (c.work_next >= j->work_end) ++j->drain_reads;
-                                              return j->drain_reads >= 2 || engine.idle(c);
-                                          }),
-                           draining.end());
-        }
-        retireFinished();
-        if (engine.idle(c)) {
-            std::lock_guard<std::mutex> lock(mu);
-            if (running.empty()) {
-                if (engine.tableSize() > 0) {
-                    rays_before += rays_now;
-                    rays_now = 0;
-                    (void)engine.resetTable();
-                    last = Counters{};
-                }
+        if (in_flight >= 0) {
+            const double t0 = msSince(ctx);
+            if (finishRound(uint32_t(in_flight), cur) != AB_OK) {
+                std::lock_guard<std::mutex> lock(mu);
+                thread_status = AB_DEVICE_ERROR;
+                break;
             }
+            busy_ms += msSince(ctx) - t0;
         }
+        in_flight = cur;
     }
     // Jobs left at shutdown are cancelled.
     std::lock_guard<std::mutex> lock(mu);
@@ -387,6 +479,13 @@ void Server::loop()
         if (j.state == kQueued || j.state == kRunning) j.state = kCancelled;
         if (j.dev) (void)hipFree(j.dev);
         j.dev = nullptr;
+    }
+    for (auto& list : to_free) {
+        for (const auto& b : list) {
+            (void)hipFree(b.first);
+            used -= b.second;
+        }
+        list.clear();
     }
     draining.clear();
     running.clear();
@@ -442,7 +541,11 @@ ab_status ab_server_stop(ab_context* ctx)
     s->cv.notify_all();
     if (s->thread.joinable()) s->thread.join();
     if (s->poll_stream) (void)hipStreamDestroy(s->poll_stream);
-    if (s->h_done) (void)hipHostFree(s->h_done);
+    for (int k = 0; k < 2; ++k) {
+        if (s->h_done[k]) (void)hipHostFree(s->h_done[k]);
+        if (s->d_done[k]) (void)hipFree(s->d_done[k]);
+    }
+    if (s->retire_stream) (void)hipStreamDestroy(s->retire_stream);
     ctx->server = nullptr;
     delete s;
     return AB_OK;

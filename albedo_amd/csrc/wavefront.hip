@@ -1475,6 +1475,33 @@ hipError_t launchResolve(const unsigned long long* accum, uint64_t npx, float* m
     return hipGetLastError();
 }
 
+namespace {
+
+struct DoneList {
+    const unsigned long long* lanes[kDoneListMax];
+};
+
+__global__ __launch_bounds__(64) void kSumDone(DoneList l, unsigned long long* out)
+{
+    // Lane k reads counter k of job blockIdx.x; the wave sums them.
+    unsigned long long v = l.lanes[blockIdx.x][size_t(threadIdx.x) * kDoneStride];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (threadIdx.x == 0) out[blockIdx.x] = v;
+}
+static_assert(kDoneLanes == 64, "kSumDone gives each of a job's counters one lane of a wave");
+
+} // namespace
+
+hipError_t launchSumDone(unsigned long long* const* done, uint32_t n, unsigned long long* out, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    if (n > kDoneListMax) return hipErrorInvalidValue;
+    DoneList l{};
+    for (uint32_t i = 0; i < n; ++i) l.lanes[i] = done[i];
+    hipLaunchKernelGGL(kSumDone, dim3(n), dim3(64), 0, s, l, out);
+    return hipGetLastError();
+}
+
 hipError_t launchAddWork(const PathState& ps, unsigned long long total, uint32_t seg_lo, uint32_t seg_hi,
                          hipStream_t s)
 {

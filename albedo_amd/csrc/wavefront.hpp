@@ -178,5 +178,9 @@ hipError_t launchAddWork(const PathState& ps, unsigned long long work_total, uin
 // Per-pixel mean radiance of a job: sums divided by the sample count, into
 // mean[npx * 3] (zero where no sample landed yet).
 hipError_t launchResolve(const unsigned long long* accum, uint64_t npx, float* mean, hipStream_t s);
+// Finished-sample counts of up to kDoneListMax jobs: out[i] = the sum of the
+// kDoneLanes counters at done[i] (a job's `done` lanes), one wave per job.
+constexpr uint32_t kDoneListMax = 64;
+hipError_t launchSumDone(unsigned long long* const* done, uint32_t n, unsigned long long* out, hipStream_t s);
 
 } // namespace ab
