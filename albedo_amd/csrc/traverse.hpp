@@ -176,6 +176,12 @@ __device__ __forceinline__ bool triHit(const ShearRay& s, const float4& v0, cons
 // Slab test of a node's two children (Ize 2013: exits widened by 1 + 2^-20, more
 // than the 1 + 2 gamma(3) rounding bound). A zero direction component is
 // replaced by +-1e-20 so no 0 * inf NaN can arise; the test stays conservative.
+// The inverse direction comes from the hardware reciprocal (v_rcp_f32, within
+// 1 ulp) instead of an IEEE division: that adds at most 1.5 ulp to each slab
+// distance, well inside the 16-ulp (2^-20) widening, so the box test stays
+// conservative and hits do not change (the test only decides which nodes are
+// visited). Rays start, enter and leave instances with it: coherent trefoil set
+// 11 459-11 588 -> 11 715-11 725 Mrays/s, full-feature scene 1.59 -> 1.56 s.
 struct BoxRay {
     float ox, oy, oz, ix, iy, iz;
 };
@@ -186,9 +192,9 @@ __device__ __forceinline__ BoxRay makeBoxRay(float ox, float oy, float oz, float
     b.ox = ox;
     b.oy = oy;
     b.oz = oz;
-    b.ix = 1.0f / (fabsf(dx) > 1e-20f ? dx : copysignf(1e-20f, dx));
-    b.iy = 1.0f / (fabsf(dy) > 1e-20f ? dy : copysignf(1e-20f, dy));
-    b.iz = 1.0f / (fabsf(dz) > 1e-20f ? dz : copysignf(1e-20f, dz));
+    b.ix = __builtin_amdgcn_rcpf(fabsf(dx) > 1e-20f ? dx : copysignf(1e-20f, dx));
+    b.iy = __builtin_amdgcn_rcpf(fabsf(dy) > 1e-20f ? dy : copysignf(1e-20f, dy));
+    b.iz = __builtin_amdgcn_rcpf(fabsf(dz) > 1e-20f ? dz : copysignf(1e-20f, dz));
     return b;
 }
 
