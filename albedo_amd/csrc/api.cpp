@@ -217,6 +217,15 @@ ab_status ab_scene_create(ab_context* ctx, const ab_scene_desc* desc, ab_scene**
     // The four-wide form covers host-built trees only.
     s->dev.wide_root = hs.wide_tlas_root;
     s->dev.wide_stack = wide_stack;
+    // Kind-split TLASes (spheres one ray per lane, then instances seeded with the
+    // sphere hit): stack bounds as for the whole TLAS; the path tracer's stack
+    // classes reach 64 entries, beyond which the scene is traced whole.
+    s->dev.sphere_stack = hs.sphere_depth ? hs.sphere_depth - 1 : 0;
+    s->dev.mesh_stack = (hs.mesh_depth ? hs.mesh_depth - 1 : 0) + (hs.max_blas_depth ? hs.max_blas_depth - 1 : 0);
+    const bool kind_split = hs.sphere_root != kNoChild && hs.mesh_root != kNoChild && s->dev.sphere_stack <= 64 &&
+                            s->dev.mesh_stack <= 64;
+    s->dev.sphere_root = kind_split ? hs.sphere_root : kNoChild;
+    s->dev.mesh_root = kind_split ? hs.mesh_root : kNoChild;
     s->dev.has_spheres = 0;
     s->dev.has_meshes = 0;
     for (const DevObject& o : hs.objects) {

@@ -61,7 +61,9 @@ void chooseRefillStack(LaunchShape& s, uint32_t depth)
 void settleBlocks(LaunchShape& s, const ab_launch_config& cfg, uint32_t capacity)
 {
     s.block = cfg.block_size;
-    if (s.block == 256 && s.pstack > 32) s.block = 128; // 256 x 64 stack entries exceed the LDS budget
+    // 256 x 64 stack entries exceed the LDS budget (the one-ray-per-lane kernels'
+    // and the sphere pass's classes).
+    if (s.block == 256 && std::max(s.pstack, s.sstack) > 32) s.block = 128;
     s.grid = gridFor(cfg.grid_blocks, capacity, s.block);
 }
 
@@ -96,12 +98,36 @@ void chooseTracing(LaunchShape& s, const SceneSet& set, uint32_t capacity, bool 
         chooseRefillStack(s, set.first.stack_depth);
         s.qgrid = qgrid;
         s.pstack = kShortStack; // no ray goes one per lane
+        if (set.first.sphere_root != kNoChild && set.first.mesh_root != kNoChild) {
+            // Kind-split: the spheres one ray per lane (their FP64 test out of the
+            // refill kernel, which then needs fewer registers and diverges less),
+            // then the instances seeded with the sphere hits.
+            s.sphere_pass = 1;
+            s.sphere_scene = set.first;
+            s.sphere_scene.tlas_root = set.first.sphere_root;
+            s.sphere_scene.stack_depth = set.first.sphere_stack;
+            s.sphere_scene.has_meshes = 0;
+            s.sstack = stackClass(set.first.sphere_stack);
+            s.scene.tlas_root = set.first.mesh_root;
+            s.scene.stack_depth = set.first.mesh_stack;
+            s.scene.has_spheres = 0;
+            s.q_spheres = 0;
+            chooseRefillStack(s, set.first.mesh_stack);
+        }
         settleBlocks(s, cfg, capacity);
     } else if (set.meshes) {
         s.refill = kTraceSplit;
         s.q_spheres = set.mesh_spheres ? 1u : 0u;
         chooseRefillStack(s, set.deepest_meshes);
         s.qgrid = qgrid;
+        if (set.kind_split) {
+            // The tagged rays of kind-split scenes: spheres one ray per lane
+            // first, then every tagged run seeded (others start from a miss).
+            s.sphere_pass = 1;
+            s.sstack = stackClass(set.deepest_sphere_pass);
+            s.q_spheres = set.unsplit_mesh_spheres ? 1u : 0u;
+            chooseRefillStack(s, set.deepest_mesh_pass);
+        }
     }
 }
 
@@ -110,7 +136,9 @@ bool sameTracing(const LaunchShape& a, const LaunchShape& b)
     if (a.refill != b.refill || a.pstack != b.pstack || a.block != b.block) return false;
     if (a.refill == kTracePlain) return true;
     return a.qgrid == b.qgrid && a.q_spheres == b.q_spheres && a.q_reg == b.q_reg && a.qstack == b.qstack &&
-           std::memcmp(&a.scene, &b.scene, sizeof(DevScene)) == 0;
+           a.sphere_pass == b.sphere_pass && a.sstack == b.sstack &&
+           std::memcmp(&a.scene, &b.scene, sizeof(DevScene)) == 0 &&
+           std::memcmp(&a.sphere_scene, &b.sphere_scene, sizeof(DevScene)) == 0;
 }
 
 #define HIP_TRY(expr, what)                                   \

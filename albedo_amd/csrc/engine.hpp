@@ -46,6 +46,11 @@ struct SceneSet {
     bool mesh_spheres{false};  // a scene with meshes also has spheres
     uint32_t deepest_meshes{0}; // largest stack bound of the scenes with meshes
     uint32_t deepest_plain{0};  // ... of the scenes without
+    // Scenes with meshes traced in two passes (kind-split, DevScene::sphere_root):
+    bool kind_split{false};            // any
+    uint32_t deepest_sphere_pass{0};   // their sphere TLASes' largest stack bound
+    uint32_t deepest_mesh_pass{0};     // the refill pass's: their instance TLASes', the others' whole
+    bool unsplit_mesh_spheres{false};  // a scene with meshes and spheres traced whole
     DevScene first{};
     void add(const DevScene& s)
     {
@@ -57,6 +62,15 @@ struct SceneSet {
             meshes = true;
             mesh_spheres = mesh_spheres || s.has_spheres;
             deepest_meshes = deepest_meshes > s.stack_depth ? deepest_meshes : s.stack_depth;
+            uint32_t pass = s.stack_depth;
+            if (s.sphere_root != kNoChild && s.mesh_root != kNoChild) {
+                kind_split = true;
+                deepest_sphere_pass = deepest_sphere_pass > s.sphere_stack ? deepest_sphere_pass : s.sphere_stack;
+                pass = s.mesh_stack;
+            } else if (s.has_spheres) {
+                unsplit_mesh_spheres = true;
+            }
+            deepest_mesh_pass = deepest_mesh_pass > pass ? deepest_mesh_pass : pass;
         }
     }
 };

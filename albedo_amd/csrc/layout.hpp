@@ -132,6 +132,16 @@ struct DevScene {
     // has no four-wide form, and the most stack entries a four-wide query holds.
     int32_t wide_root;
     uint32_t wide_stack;
+    // Scenes with both spheres and mesh instances also hold a TLAS over the
+    // spheres alone and one over the instances alone (kNoChild otherwise), and
+    // the most stack entries a query over each holds (the instances' with their
+    // BLASes). The path tracer traces such a scene's rays in two passes: the
+    // spheres one ray per lane, then the instances with lane refill, seeded with
+    // the nearest sphere hit.
+    int32_t sphere_root;
+    int32_t mesh_root;
+    uint32_t sphere_stack;
+    uint32_t mesh_stack;
 };
 
 } // namespace ab

@@ -798,6 +798,22 @@ ab_status buildHostScene(const ab_scene_desc& d, HostScene& out, std::string& er
         out.tlas_nodes = out.nodes.size() - before;
         out.tlas_depth = builder.depth_reached + 1;
         if (wide) out.wide_tlas_root = flattenWide(builder, root, leaf_base, out.nodes, out.wide_tlas_stack);
+        // The same objects split by kind, when both kinds are present.
+        std::vector<Item> spheres_only, instances_only;
+        for (const Item& it : items)
+            (out.objects[it.id].kind == kObjSphere ? spheres_only : instances_only).push_back(it);
+        if (!spheres_only.empty() && !instances_only.empty()) {
+            auto build_part = [&](std::vector<Item>& part, int32_t& part_root, uint32_t& part_depth) {
+                SahBuilder b(kCtInstances);
+                const int32_t r = b.build(part);
+                const uint32_t base = uint32_t(out.leafobj.size());
+                for (const Item& it : part) out.leafobj.push_back(it.id);
+                part_root = flatten(b, r, base, out.nodes);
+                part_depth = b.depth_reached + 1;
+            };
+            build_part(spheres_only, out.sphere_root, out.sphere_depth);
+            build_part(instances_only, out.mesh_root, out.mesh_depth);
+        }
     }
     out.max_blas_depth = blas_depth;
     out.hash = hashSceneDesc(d);

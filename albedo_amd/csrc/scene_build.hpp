@@ -33,6 +33,12 @@ struct HostScene {
     // Four-wide form of the same trees (DevScene::wide_root): its TLAS root and
     // the most stack entries its TLAS and its deepest BLAS need.
     int32_t wide_tlas_root{kNoChild};
+    // With both spheres and instances: TLASes over each kind alone (DevScene::
+    // sphere_root, mesh_root) and their levels.
+    int32_t sphere_root{kNoChild};
+    int32_t mesh_root{kNoChild};
+    uint32_t sphere_depth{0};
+    uint32_t mesh_depth{0};
     uint32_t wide_tlas_stack{0};
     uint32_t wide_blas_stack{0};
     int32_t sky_light{-1};

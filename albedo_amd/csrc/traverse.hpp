@@ -482,7 +482,18 @@ struct RayQuery {
         pending = kNoChild;
         sp = sp_blas = 0;
         in_blas = false;
-        done = ref == kNoChild;
+        // An empty range (tmax <= tmin, or a NaN bound) admits no hit.
+        done = ref == kNoChild || !(r.tmax > r.tmin);
+    }
+
+    // Starts from a hit found over other objects of the scene (a closest-hit
+    // query continued over the rest): the answer is then the lexicographic
+    // minimum over both sets, and boxes beyond the hit are culled from the start.
+    __device__ __forceinline__ void seed(const HitF& h)
+    {
+        if (h.object == kMissId) return;
+        best = h;
+        lim = cullLimit(h.t);
     }
 
     __device__ __forceinline__ bool found() const { return ANY && best.object != kMissId; }

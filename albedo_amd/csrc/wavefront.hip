@@ -1039,6 +1039,70 @@ __device__ __forceinline__ RayF queuedRay(const PathState& ps, uint32_t slot)
     return RayF{o.x, o.y, o.z, 0.0f, d.x, d.y, d.z, ANY ? o.w : __builtin_huge_valf()};
 }
 
+// First pass over a kind-split scene (`sph` holds its sphere-only TLAS), one ray
+// per lane over the queue the refill kernel then traces: a closest-hit ray
+// leaves its nearest sphere hit (or a miss) in the path's hit, which the
+// instance pass starts from; a shadow ray a sphere occludes gets an empty range
+// and no contribution, so the instance pass adds nothing for it. The objects,
+// materials and arrays are the scene's own, so ids and keys are unchanged.
+template <int BLOCK, int STACK, bool ANY>
+__global__ __launch_bounds__(BLOCK) void kSpherePass(PathState ps, DevScene sph)
+{
+    __shared__ int32_t stack[STACK * BLOCK];
+    const Counters* c = ps.ctr;
+    const uint32_t n = ANY ? c->n_shadow : c->n_active + pendingGeneration(c).n_extra;
+    const LdsStack<BLOCK> stk{stack + threadIdx.x};
+    for (uint32_t k = gridStart(); k < n; k += gridStride()) {
+        const uint32_t slot = (ANY ? ps.shadowq[k] : ps.active[k]) & kSlotMask;
+        const RayF r = queuedRay<ANY>(ps, slot);
+        HitF h;
+        const bool hit = traceRay<BLOCK, ANY>(sph, r, h, stk);
+        if (ANY) {
+            if (hit) {
+                const float4 o = ps.sh_org[slot];
+                ps.sh_org[slot] = make_float4(o.x, o.y, o.z, 0.0f);
+                ps.sh_c[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+        } else {
+            ps.hit[slot] = make_uint4(__float_as_uint(h.t), h.object, h.prim, 0u);
+        }
+    }
+}
+
+// The same first pass over the tagged range of a tag-sorted queue (split
+// tracing): each ray's scene comes from its entry's tag; a ray of a kind-split
+// scene is traced over that scene's sphere TLAS, the rays of other scenes get a
+// miss (a closest-hit ray must not start from its previous bounce's hit).
+template <int BLOCK, int STACK, bool ANY>
+__global__ __launch_bounds__(BLOCK) void kSpherePassTagged(PathState ps)
+{
+    __shared__ int32_t stack[STACK * BLOCK];
+    const Counters* c = ps.ctr;
+    const uint32_t first = ANY ? c->sh_untagged : c->n_untagged;
+    const uint32_t n = ANY ? c->n_shadow : c->n_active + pendingGeneration(c).n_extra;
+    const LdsStack<BLOCK> stk{stack + threadIdx.x};
+    for (uint32_t k = first + gridStart(); k < n; k += gridStride()) {
+        const uint32_t entry = ANY ? ps.shadowq[k] : ps.active[ps.bytag[k]];
+        const uint32_t slot = entry & kSlotMask;
+        DevScene sc = ps.tag_scenes[entry >> kSlotBits];
+        HitF h{__builtin_huge_valf(), kMissId, kMissId, 0.0f, 0.0f};
+        bool hit = false;
+        if (sc.sphere_root != kNoChild) {
+            sc.tlas_root = sc.sphere_root;
+            hit = traceRay<BLOCK, ANY>(sc, queuedRay<ANY>(ps, slot), h, stk);
+        }
+        if (ANY) {
+            if (hit) {
+                const float4 o = ps.sh_org[slot];
+                ps.sh_org[slot] = make_float4(o.x, o.y, o.z, 0.0f);
+                ps.sh_c[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+        } else {
+            ps.hit[slot] = make_uint4(__float_as_uint(h.t), h.object, h.prim, 0u);
+        }
+    }
+}
+
 // Waves per SIMD the refill kernels ask the compiler for: with the FP64 sphere
 // test 6 (80 VGPRs, which with the 24-entry stack class's 6 KB of LDS per wave
 // leaves room for 6 waves), without it 7 (72 VGPRs).
@@ -1046,7 +1110,10 @@ template <bool SPHERES>
 constexpr int refillWaves() { return SPHERES ? 6 : 7; }
 
 // Every ray of the queue over one scene, passed as a kernel argument.
-template <int STACK, bool ANY, bool SPHERES, bool LEAF_IN_REG>
+// SEEDED: the second pass of a kind-split scene (`sc` holds the instance-only
+// TLAS): closest-hit queries start from the sphere hit kSpherePass left in the
+// path's hit; shadow rays that pass found occluded arrive with an empty range.
+template <int STACK, bool ANY, bool SPHERES, bool LEAF_IN_REG, bool SEEDED>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(refillWaves<SPHERES>(), 8))) void kTraceQueue(
     PathState ps, DevScene sc)
 {
@@ -1071,6 +1138,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(refillWaves<
             if (idle && mine < end) {
                 slot = (ANY ? ps.shadowq[mine] : ps.active[mine]) & kSlotMask;
                 q.begin(sc, queuedRay<ANY>(ps, slot));
+                if (SEEDED && !ANY) {
+                    const uint4 h = ps.hit[slot];
+                    q.seed(HitF{__uint_as_float(h.x), h.y, h.z, 0.0f, 0.0f});
+                }
                 id = mine;
             }
         }
@@ -1121,6 +1192,10 @@ __device__ __forceinline__ DevScene sceneOfTag(const PathState& ps, uint32_t tag
     u.tri_count = waveUniform(s.tri_count);
     u.wide_root = int32_t(waveUniform(uint32_t(s.wide_root)));
     u.wide_stack = waveUniform(s.wide_stack);
+    u.sphere_root = int32_t(waveUniform(uint32_t(s.sphere_root)));
+    u.mesh_root = int32_t(waveUniform(uint32_t(s.mesh_root)));
+    u.sphere_stack = waveUniform(s.sphere_stack);
+    u.mesh_stack = waveUniform(s.mesh_stack);
     return u;
 }
 
@@ -1133,7 +1208,11 @@ __device__ __forceinline__ DevScene sceneOfTag(const PathState& ps, uint32_t tag
 // take only the following rays of the same tag and stop at the first ray of
 // another. Traversal then reads one scene per wave from scalar registers, as
 // kTraceQueue does with its kernel argument, instead of a scene per lane.
-template <int STACK, bool ANY, bool SPHERES, bool LEAF_IN_REG>
+// SEEDED: a kind-split scene's run walks its instance-only TLAS, starting from
+// the sphere hit kSpherePassTagged left (closest hits; shadow rays a sphere
+// occluded arrive with an empty range); other scenes' runs walk their whole
+// TLAS, and the pass left them a miss.
+template <int STACK, bool ANY, bool SPHERES, bool LEAF_IN_REG, bool SEEDED>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(refillWaves<SPHERES>(), 8))) void kTraceGrouped(
     PathState ps)
 {
@@ -1175,6 +1254,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(refillWaves<
                     start = uint32_t(__ffsll(static_cast<long long>(owned))) - 1;
                     run_tag = waveUniform(uint32_t(__shfl(w_tag, int(start))));
                     sc = sceneOfTag(ps, run_tag);
+                    if (SEEDED && sc.mesh_root != kNoChild) {
+                        sc.tlas_root = sc.mesh_root;
+                        sc.has_spheres = 0;
+                    }
                 }
             }
             if (fill) {
@@ -1186,6 +1269,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(refillWaves<
                 if (idle && k < take) {
                     slot = k_entry & kSlotMask;
                     q.begin(sc, queuedRay<ANY>(ps, slot));
+                    if (SEEDED && !ANY) {
+                        const uint4 h = ps.hit[slot];
+                        q.seed(HitF{__uint_as_float(h.x), h.y, h.z, 0.0f, 0.0f});
+                    }
                     id = k_pos;
                 }
                 next += take;
@@ -1357,8 +1444,37 @@ template <int STACK, bool ANY, bool SPHERES, bool REG>
 void launchRefill(const PathState& ps, const LaunchShape& sh, hipStream_t s)
 {
     const dim3 g(sh.qgrid), b(64);
-    if (sh.refill == kTraceSplit) hipLaunchKernelGGL((kTraceGrouped<STACK, ANY, SPHERES, REG>), g, b, 0, s, ps);
-    else hipLaunchKernelGGL((kTraceQueue<STACK, ANY, SPHERES, REG>), g, b, 0, s, ps, sh.scene);
+    if (sh.refill == kTraceSplit) {
+        if (sh.sphere_pass) hipLaunchKernelGGL((kTraceGrouped<STACK, ANY, SPHERES, REG, true>), g, b, 0, s, ps);
+        else hipLaunchKernelGGL((kTraceGrouped<STACK, ANY, SPHERES, REG, false>), g, b, 0, s, ps);
+    } else if (!SPHERES && sh.sphere_pass)
+        hipLaunchKernelGGL((kTraceQueue<STACK, ANY, false, REG, true>), g, b, 0, s, ps, sh.scene);
+    else hipLaunchKernelGGL((kTraceQueue<STACK, ANY, SPHERES, REG, false>), g, b, 0, s, ps, sh.scene);
+}
+
+// The sphere pass of a kind-split scene, in the stack class of its sphere TLAS
+// (blocks of 256 take classes up to 32 entries, as the one-ray-per-lane kernels).
+template <int BLOCK, bool ANY>
+hipError_t launchSpherePass(const PathState& ps, const LaunchShape& sh, hipStream_t s)
+{
+    const dim3 g(sh.grid), b(BLOCK);
+    const bool split = sh.refill == kTraceSplit;
+    if (sh.sstack <= kShortStack) {
+        if (split) hipLaunchKernelGGL((kSpherePassTagged<BLOCK, kShortStack, ANY>), g, b, 0, s, ps);
+        else hipLaunchKernelGGL((kSpherePass<BLOCK, kShortStack, ANY>), g, b, 0, s, ps, sh.sphere_scene);
+    } else if (sh.sstack <= 32) {
+        if (split) hipLaunchKernelGGL((kSpherePassTagged<BLOCK, 32, ANY>), g, b, 0, s, ps);
+        else hipLaunchKernelGGL((kSpherePass<BLOCK, 32, ANY>), g, b, 0, s, ps, sh.sphere_scene);
+    } else {
+        if constexpr (BLOCK < 256) {
+            if (sh.sstack > 64) return hipErrorInvalidValue;
+            if (split) hipLaunchKernelGGL((kSpherePassTagged<BLOCK, 64, ANY>), g, b, 0, s, ps);
+            else hipLaunchKernelGGL((kSpherePass<BLOCK, 64, ANY>), g, b, 0, s, ps, sh.sphere_scene);
+        } else {
+            return hipErrorInvalidValue;
+        }
+    }
+    return hipGetLastError();
 }
 
 template <int STACK, bool ANY>
@@ -1401,6 +1517,10 @@ hipError_t bounce(const PathState& ps, const LaunchShape& sh, hipStream_t s)
         hipLaunchKernelGGL((kExtend<BLOCK, STACK, false>), dim3(grid), dim3(BLOCK), 0, s, ps);
     } else {
         if (split) hipLaunchKernelGGL((kExtend<BLOCK, STACK, true>), dim3(grid), dim3(BLOCK), 0, s, ps);
+        if (sh.sphere_pass) {
+            const hipError_t e = launchSpherePass<BLOCK, false>(ps, sh, s);
+            if (e != hipSuccess) return e;
+        }
         launchTraceQueue<false>(ps, sh, s);
         hipLaunchKernelGGL((kCountBins<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     }
@@ -1418,6 +1538,10 @@ hipError_t bounce(const PathState& ps, const LaunchShape& sh, hipStream_t s)
         hipLaunchKernelGGL((kShadow<BLOCK, STACK, false>), dim3(grid), dim3(BLOCK), 0, s, ps);
     else if (split)
         hipLaunchKernelGGL((kShadow<BLOCK, STACK, true>), dim3(grid), dim3(BLOCK), 0, s, ps);
+    if (sh.sphere_pass) {
+        const hipError_t e = launchSpherePass<BLOCK, true>(ps, sh, s);
+        if (e != hipSuccess) return e;
+    }
     if (sh.refill != kTracePlain) launchTraceQueue<true>(ps, sh, s);
     hipLaunchKernelGGL((kFinish<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     return hipGetLastError();

@@ -160,7 +160,13 @@ struct LaunchShape {
     uint32_t qstack{0};    // stack class of the lane-refill kernels (20, 24, 32 or 64 entries)
     uint32_t q_spheres{0}; // a scene the refill kernels trace has spheres (the FP64 sphere test is compiled in)
     uint32_t q_reg{0};     // such a scene's stack bound + 1 exceeds the class: TLAS leaf rest in a register
-    DevScene scene{};      // kTraceOne: the scene
+    DevScene scene{};      // kTraceOne: the scene (its instance-only TLAS with sphere_pass)
+    // kTraceOne over a scene with both spheres and instances: kSpherePass traces
+    // `sphere_scene` (its sphere-only TLAS, stack class `sstack`) before the
+    // refill kernel traces `scene` seeded with the sphere hits.
+    uint32_t sphere_pass{0};
+    uint32_t sstack{0};
+    DevScene sphere_scene{};
 };
 
 // Enqueues two bounces (generate, extend, bin, shade, shadow, finish, with the
