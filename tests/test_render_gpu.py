@@ -379,3 +379,31 @@ def test_batched_job_mix_equals_renders_alone(ctx):
         total += w * h
     print("room variants: %d of %d pixels off by more than 0.1" % (off, total))
     assert off < 0.01 * total
+
+
+def test_batch_with_more_mesh_scenes_than_tags_equals_renders_alone(ctx):
+    """66 scenes with meshes in one batch: the first 63 get scene tags (their rays
+    traced a scene per wave, the kind-split ones after a sphere pass), the rest
+    none (their rays go one per lane over the whole TLAS, in the stack class of
+    the untagged scenes, which then includes theirs). Half the scenes also hold a
+    sphere. Every image equals the job rendered alone."""
+    from albedo_amd import render_batch
+    from albedo_amd.scene import MeshGeometry, MeshInstance, Transform
+    knot = MeshGeometry(W.trefoil_mesh(segments=24, sides=6), 1)
+    jobs = []
+    for k in range(66):
+        s = Scene()
+        s.add(MeshInstance(knot, Transform.translation((0.05 * k, 0.0, 0.0))),
+              DiffuseMaterial(ConstantTexture((0.7, 0.6 - 0.005 * k, 0.5))))
+        if k % 2:
+            s.add(Sphere((2.0, 1.5, -1.0), 0.8), DiffuseMaterial(ConstantTexture((0.3, 0.5, 0.7))))
+        s.add_light(PointLight((2.0, 4.0, 9.0), (60.0, 60.0, 60.0)))
+        s.set_sky(SkyLight.uniform((0.1, 0.1, 0.12)))
+        cam = CameraSettings((0.0, 0.0, 9.0), (0.05 * k, 0.0, 0.0), (0, 1, 0), 40.0, 16 / 12)
+        jobs.append((ctx.scene(s), cam, 16, 12, 1 + k % 2, 5, 2, 100 + k))
+    outs, st = render_batch(ctx, jobs)
+    assert st["camera_rays"] == sum(j[2] * j[3] * j[4] ** 2 for j in jobs)
+    assert all(float(o.max()) > 0.0 for o in outs)
+    for j, out in zip(jobs, outs):
+        alone = j[0].render(*j[1:])
+        assert np.array_equal(alone.view(np.uint32), out.view(np.uint32)), j[2:]
