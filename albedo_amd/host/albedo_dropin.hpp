@@ -3,13 +3,15 @@
 //
 // Mirrors, by name and meaning:
 //   Transform                      core/transform.hpp:38-93
-//   Sphere, TriangleMesh           geometry/sphere.hpp, geometry/triangle_mesh.hpp
+//   Sphere, Triangle, TriangleMesh geometry/sphere.hpp, geometry/triangle.hpp,
+//                                  geometry/triangle_mesh.hpp
 //   MeshGeometry, MeshShading      geometry/mesh_geometry.hpp:21-66
 //   MeshInstance                   geometry/mesh_instance.hpp:18-66
 //   Constant/Checker/PerlinTexture texture/*.hpp
 //   Diffuse/Conductor/Dielectric/MixMaterial, PerlinPattern  material/*.hpp
 //   PointLight, SkyLight           light/point_light.hpp, light/sky_light.hpp
-//   Scene                          scene/scene.hpp:31-98 (add, addAreaLight, addLight, setSky)
+//   Scene                          scene/scene.hpp:31-98 (add of a shape or a TriangleMesh,
+//                                  addAreaLight, addLight, setSky, shapeCount)
 //   CameraSettings, PerspectiveCamera  camera/perspective_camera.hpp:16-53
 //   PathEstimator                  estimator/path_estimator.hpp:22-38
 //   RenderJob, renderImage         render/render_image.hpp:17-49
@@ -51,6 +53,9 @@ struct Point2r {
 };
 struct Point2i {
     int x{}, y{};
+};
+struct Normal3r {
+    Real x{}, y{}, z{};
 };
 struct Rgb {
     Real r{}, g{}, b{};
@@ -234,6 +239,50 @@ private:
     Real r_;
 };
 
+// True when Triangle{p0, p1, p2} would not throw: finite coordinates and an edge
+// cross product whose squared length is a normal number (reference
+// geometry/triangle.cpp:198-205).
+inline bool formsTriangle(Point3r p0, Point3r p1, Point3r p2) noexcept
+{
+    for (const Point3r& p : {p0, p1, p2})
+        if (!std::isfinite(p.x) || !std::isfinite(p.y) || !std::isfinite(p.z)) return false;
+    const Real e1[3] = {p1.x - p0.x, p1.y - p0.y, p1.z - p0.z}, e2[3] = {p2.x - p0.x, p2.y - p0.y, p2.z - p0.z};
+    const Real n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    return std::isnormal(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+}
+
+// A world-space triangle with per-corner texture coordinates (reference
+// geometry/triangle.hpp:22-78). The device traces it as a one-face flat mesh under
+// the identity map, which intersects it exactly as the same face of any mesh.
+class Triangle {
+public:
+    Triangle(Point3r p0, Point3r p1, Point3r p2,
+             const std::array<Point2r, 3>& uv = {Point2r{0, 0}, Point2r{1, 0}, Point2r{1, 1}})
+        : corners_{p0, p1, p2}, uv_(uv)
+    {
+        for (const Point3r& p : corners_)
+            if (!std::isfinite(p.x) || !std::isfinite(p.y) || !std::isfinite(p.z))
+                throw std::invalid_argument("triangle vertices must have finite coordinates");
+        if (!formsTriangle(p0, p1, p2)) throw std::invalid_argument("triangle has no computable area");
+        for (const Point2r& t : uv_)
+            if (!std::isfinite(t.x) || !std::isfinite(t.y))
+                throw std::invalid_argument("triangle texture coordinates must be finite");
+        const Real e1[3] = {p1.x - p0.x, p1.y - p0.y, p1.z - p0.z}, e2[3] = {p2.x - p0.x, p2.y - p0.y, p2.z - p0.z};
+        const Real n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                           e1[0] * e2[1] - e1[1] * e2[0]};
+        const Real l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        normal_ = {n[0] / l, n[1] / l, n[2] / l};
+    }
+    const std::array<Point3r, 3>& vertices() const noexcept { return corners_; }
+    const std::array<Point2r, 3>& textureCoordinates() const noexcept { return uv_; }
+    Normal3r normal() const noexcept { return normal_; }
+
+private:
+    std::array<Point3r, 3> corners_;
+    std::array<Point2r, 3> uv_;
+    Normal3r normal_{};
+};
+
 // Positions and triangle faces, with optional per-corner texture coordinates.
 class TriangleMesh {
 public:
@@ -242,9 +291,16 @@ public:
                  std::vector<Face> uv_faces = {})
         : positions_(std::move(positions)), faces_(std::move(faces)), uv_(std::move(uv)), uv_faces_(std::move(uv_faces))
     {
+        // Checks in the reference's order (triangle_mesh.cpp:60-74).
+        for (const Point3r& p : positions_)
+            if (!std::isfinite(p.x) || !std::isfinite(p.y) || !std::isfinite(p.z))
+                throw std::invalid_argument("mesh position has a non-finite coordinate");
         for (const Face& f : faces_)
             for (std::size_t i : f)
                 if (i >= positions_.size()) throw std::out_of_range("mesh face uses a position the mesh does not have");
+        for (const Point2r& t : uv_)
+            if (!std::isfinite(t.x) || !std::isfinite(t.y))
+                throw std::invalid_argument("mesh texture coordinate is not finite");
         if (!uv_faces_.empty() && uv_faces_.size() != faces_.size())
             throw std::invalid_argument("mesh has a different number of texture faces and faces");
         for (const Face& f : uv_faces_)
@@ -343,7 +399,7 @@ private:
     Transform to_world_;
 };
 
-using Shape = std::variant<Sphere, MeshInstance>;
+using Shape = std::variant<Sphere, Triangle, MeshInstance>;
 
 struct ConstantTexture {
     Rgb color{};
@@ -399,7 +455,23 @@ struct SkyLight {
 // Objects and lights; flattens into an ab_scene_desc when rendered.
 class Scene {
 public:
-    void add(const Shape& shape, const Material& material) { objects_.push_back({shape, material, -1}); }
+    void add(const Shape& shape, const Material& material) { objects_.push_back({shape, material, -1, 1}); }
+
+    // Every triangle of `mesh` with a computable area, in face order, with one
+    // material (reference scene/scene.cpp:33-60). They become one flat mesh under
+    // the identity map: the same faces hit exactly as separate triangles would,
+    // and the (t, object, face) tie order is the same because the faces keep their
+    // order and stay together in the object order. A mesh with no such triangle
+    // adds nothing.
+    void add(const TriangleMesh& mesh, const Material& material)
+    {
+        std::size_t kept = 0;
+        const auto& P = mesh.positions();
+        for (const auto& f : mesh.faces()) kept += formsTriangle(P[f[0]], P[f[1]], P[f[2]]) ? 1 : 0;
+        if (kept == 0) return;
+        auto geometry = std::make_shared<const MeshGeometry>(mesh, MeshShading::Flat);
+        objects_.push_back({MeshInstance{std::move(geometry), Transform{}}, material, -1, kept});
+    }
 
     void addAreaLight(const Sphere& sphere, Rgb radiance, const Material& material = DiffuseMaterial{})
     {
@@ -409,7 +481,7 @@ public:
         l.object = uint32_t(objects_.size());
         l.radiance = radiance;
         lights_.push_back(l);
-        objects_.push_back({sphere, material, light});
+        objects_.push_back({sphere, material, light, 1});
     }
 
     std::size_t addLight(const PointLight& light)
@@ -434,7 +506,14 @@ public:
         }
     }
 
-    std::size_t shapeCount() const { return objects_.size(); }
+    // Objects as the reference counts them: one per shape, one per kept triangle
+    // of an added TriangleMesh.
+    std::size_t shapeCount() const
+    {
+        std::size_t n = 0;
+        for (const Object& o : objects_) n += o.count;
+        return n;
+    }
 
     // The resident device scene (built on first use, shared by content hash).
     ab_scene* device() const
@@ -452,6 +531,7 @@ private:
         Shape shape;
         Material material;
         int light;
+        std::size_t count;
     };
     struct Light {
         uint32_t kind{};
@@ -516,6 +596,16 @@ private:
         std::vector<ab_mesh_desc> meshes;
         std::vector<const MeshGeometry*> mesh_ptrs;
         std::vector<ab_object_desc> objs;
+        // One-face meshes of the Triangle shapes, kept alive until the scene is created.
+        std::vector<std::array<float, 9>> tri_pos;
+        std::vector<std::array<float, 6>> tri_uv;
+        std::size_t n_tri = 0;
+        for (const Object& o : objects_) n_tri += std::holds_alternative<Triangle>(o.shape) ? 1 : 0;
+        tri_pos.reserve(n_tri);
+        tri_uv.reserve(n_tri);
+        static const uint32_t kOneFace[3] = {0, 1, 2};
+        double identity_fwd[12], identity_inv[12];
+        Transform{}.rows(identity_fwd, identity_inv);
         for (const Object& o : objects_) {
             ab_object_desc od{};
             od.light = o.light;
@@ -547,6 +637,30 @@ private:
                 od.center[1] = s->center().y;
                 od.center[2] = s->center().z;
                 od.radius = s->radius();
+            } else if (const auto* t = std::get_if<Triangle>(&o.shape)) {
+                od.kind = AB_OBJ_INSTANCE;
+                std::array<float, 9>& p = tri_pos.emplace_back();
+                std::array<float, 6>& w = tri_uv.emplace_back();
+                for (int k = 0; k < 3; ++k) {
+                    const Point3r v = t->vertices()[k];
+                    p[3 * k] = float(v.x);
+                    p[3 * k + 1] = float(v.y);
+                    p[3 * k + 2] = float(v.z);
+                    w[2 * k] = float(t->textureCoordinates()[k].x);
+                    w[2 * k + 1] = float(t->textureCoordinates()[k].y);
+                }
+                ab_mesh_desc md{};
+                md.positions = p.data();
+                md.num_vertices = 3;
+                md.indices = kOneFace;
+                md.num_faces = 1;
+                md.uv = w.data();
+                md.smooth = 0;
+                od.mesh = uint32_t(meshes.size());
+                meshes.push_back(md);
+                mesh_ptrs.push_back(nullptr);
+                std::memcpy(od.to_world, identity_fwd, sizeof identity_fwd);
+                std::memcpy(od.to_object, identity_inv, sizeof identity_inv);
             } else {
                 const auto& inst = std::get<MeshInstance>(o.shape);
                 od.kind = AB_OBJ_INSTANCE;

@@ -177,6 +177,46 @@ class Sphere:
             raise ValueError("sphere center must be finite")
 
 
+DEFAULT_TRIANGLE_UV = ((0.0, 0.0), (1.0, 0.0), (1.0, 1.0))
+
+
+def forms_triangle(p0, p1, p2) -> bool:
+    """True when Triangle(p0, p1, p2) would not raise: finite coordinates and an
+    edge cross product whose squared length is a normal number (reference
+    geometry/triangle.cpp:198-205)."""
+    p = np.array([p0, p1, p2], dtype=np.float64)
+    if not np.all(np.isfinite(p)):
+        return False
+    with np.errstate(over="ignore", invalid="ignore"):
+        n = np.cross(p[1] - p[0], p[2] - p[0])
+        l2 = float((n * n).sum())
+    return math.isfinite(l2) and l2 >= 2.2250738585072014e-308
+
+
+@dataclass
+class Triangle:
+    """A world-space triangle with per-corner texture coordinates (reference
+    geometry/triangle.hpp:22-78). Traced as a one-face flat mesh under the
+    identity map, which intersects it exactly as the same face of any mesh."""
+    p0: Vec3
+    p1: Vec3
+    p2: Vec3
+    uv: Tuple[Tuple[float, float], ...] = DEFAULT_TRIANGLE_UV
+
+    def __post_init__(self):
+        if not np.all(np.isfinite(np.array([self.p0, self.p1, self.p2], dtype=np.float64))):
+            raise ValueError("triangle vertices must have finite coordinates")
+        if not forms_triangle(self.p0, self.p1, self.p2):
+            raise ValueError("triangle has no computable area")
+        if len(self.uv) != 3 or not np.all(np.isfinite(np.array(self.uv, dtype=np.float64))):
+            raise ValueError("triangle texture coordinates must be three finite pairs")
+
+    def normal(self):
+        p = np.array([self.p0, self.p1, self.p2], dtype=np.float64)
+        n = np.cross(p[1] - p[0], p[2] - p[0])
+        return tuple(n / np.sqrt((n * n).sum()))
+
+
 class TriangleMesh:
     """Positions + triangle faces (+ optional per-face corner uv). Immutable once
     built (its arrays are read-only), so derived sets are computed once."""
@@ -280,12 +320,24 @@ class Scene:
         self.objects: List[tuple] = []  # (shape, material, light index or -1)
         self.lights: List[tuple] = []   # ("point", PointLight) | ("area", object) | ("sky", SkyLight)
         self.sky_index: Optional[int] = None
+        self._mesh_counts = {}  # object index -> triangles of an added TriangleMesh
 
     def add(self, shape, material: Material):
-        if isinstance(shape, (Sphere, MeshInstance)):
+        """Adds a Sphere, Triangle or MeshInstance as one object, or every
+        triangle of a TriangleMesh with a computable area, in face order
+        (reference scene/scene.cpp:25-60). The mesh's triangles become one flat
+        mesh under the identity map: the same faces hit exactly as separate
+        triangles would, and the (t, object, face) tie order is the same because
+        the faces keep their order and stay together in the object order. A mesh
+        with no such triangle adds nothing."""
+        if isinstance(shape, (Sphere, Triangle, MeshInstance)):
             self.objects.append((shape, material, -1))
+        elif isinstance(shape, TriangleMesh):
+            if len(shape.kept_faces()):
+                self.objects.append((MeshInstance(MeshGeometry(shape, FLAT), Transform()), material, -1))
+                self._mesh_counts[len(self.objects) - 1] = len(shape.kept_faces())
         else:
-            raise TypeError("Scene.add takes a Sphere or a MeshInstance")
+            raise TypeError("Scene.add takes a Sphere, Triangle, MeshInstance or TriangleMesh")
 
     def add_area_light(self, sphere: Sphere, radiance: Vec3, material: Material = None):
         if material is None:
@@ -307,7 +359,9 @@ class Scene:
             self.lights[self.sky_index] = ("sky", sky)
 
     def shape_count(self) -> int:
-        return len(self.objects)
+        """Objects as the reference counts them: one per shape, one per kept
+        triangle of an added TriangleMesh."""
+        return len(self.objects) - len(self._mesh_counts) + sum(self._mesh_counts.values())
 
     # ------------------------------------------------------------ flattening
 
@@ -403,6 +457,21 @@ class SceneDescription:
             mesh_index[id(g)] = len(meshes) - 1
             return mesh_index[id(g)]
 
+        def one_face(t: Triangle) -> int:
+            pos = np.array([t.p0, t.p1, t.p2], dtype=np.float32)
+            fac = np.array([0, 1, 2], dtype=np.uint32)
+            uv = np.array(t.uv, dtype=np.float32).reshape(6)
+            self._keep += [pos, fac, uv]
+            md = _abi.MeshDesc()
+            md.positions = pos.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+            md.num_vertices = 3
+            md.indices = fac.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+            md.num_faces = 1
+            md.uv = uv.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+            md.smooth = 0
+            meshes.append(md)
+            return len(meshes) - 1
+
         objs: List[_abi.ObjectDesc] = []
         for shape, mat, light in scene.objects:
             o = _abi.ObjectDesc()
@@ -412,6 +481,11 @@ class SceneDescription:
                 o.kind = _abi.OBJ_SPHERE
                 o.center[:] = [float(c) for c in shape.center]
                 o.radius = float(shape.radius)
+            elif isinstance(shape, Triangle):
+                o.kind = _abi.OBJ_INSTANCE
+                o.mesh = one_face(shape)
+                o.to_world[:] = Transform().rows(False)
+                o.to_object[:] = Transform().rows(True)
             else:
                 o.kind = _abi.OBJ_INSTANCE
                 o.mesh = mesh(shape.geometry)

@@ -168,6 +168,55 @@ int main(int argc, char** argv)
     std::printf("normals in [%.3f, %.3f], scatter-only mean %.6f\n", nmin, nmax, ssum / (3.0 * width * height));
     if (nmin < 0 || nmax > 1 || !(ssum > 0)) return 1;
 
+    // World-space triangles (reference geometry/triangle.hpp, scene/scene.cpp:33-60):
+    // a tent of four faces, one of them without area, added as a mesh, as
+    // Triangle shapes and as an identity MeshInstance renders the same image.
+    const TriangleMesh tent{{{-1, 0.02, 0}, {0, 1.2, -0.4}, {1, 0.02, 0}, {0, 0.02, -1.5}, {2, 0.02, 0}},
+                            {{0, 1, 2}, {0, 2, 4}, {2, 1, 3}, {3, 1, 0}}};
+    const Material red{DiffuseMaterial{CheckerTexture{{0.8, 0.2, 0.2}, {0.2, 0.2, 0.8}, 3.0}}};
+    Scene as_mesh, as_triangles, as_instance;
+    for (Scene* s : {&as_mesh, &as_triangles, &as_instance}) {
+        s->add(Sphere{{0.0, -1e4, 0.0}, 1e4}, white);
+        s->addLight(PointLight{{0.8, 4.2, 3.2}, {6.0, 6.0, 6.0}});
+        s->setSky(SkyLight::uniform({0.2, 0.25, 0.3}));
+    }
+    as_mesh.add(tent, red);
+    for (const auto& f : tent.faces()) {
+        const auto& P = tent.positions();
+        if (formsTriangle(P[f[0]], P[f[1]], P[f[2]])) as_triangles.add(Triangle{P[f[0]], P[f[1]], P[f[2]]}, red);
+    }
+    as_instance.add(MeshInstance{std::make_shared<const MeshGeometry>(tent, MeshShading::Flat), Transform{}}, red);
+    for (Scene* s : {&as_mesh, &as_triangles, &as_instance}) s->add(Sphere{{0.6, 0.5, 0.8}, 0.4}, white);
+    PixelBuffer img_mesh{width, height}, img_tris{width, height}, img_inst{width, height};
+    const PerspectiveCamera tent_camera{CameraSettings{{0.5, 2.0, 4.0}, {0.0, 0.4, -0.4}, {0, 1, 0}, 45.0,
+                                                       double(width) / height}};
+    renderImage(RenderJob{tent_camera, as_mesh, 3, 9, PathEstimator{8, 3}, 1}, img_mesh);
+    renderImage(RenderJob{tent_camera, as_triangles, 3, 9, PathEstimator{8, 3}, 1}, img_tris);
+    renderImage(RenderJob{tent_camera, as_instance, 3, 9, PathEstimator{8, 3}, 1}, img_inst);
+    const bool forms_equal = sameImage(img_mesh, img_tris) && sameImage(img_mesh, img_inst) &&
+                             !sameImage(img_mesh, by_default);
+    const bool counts_ok = as_mesh.shapeCount() == 5 && as_triangles.shapeCount() == 5 && as_instance.shapeCount() == 3;
+    std::printf("triangle_forms_equal %d\nshape_counts_ok %d\n", forms_equal ? 1 : 0, counts_ok ? 1 : 0);
+    int triangle_errors = 0;
+    const Point3r o{0, 0, 0}, x{1, 0, 0}, y{0, 1, 0};
+    const Real nan = std::nan("");
+    for (const std::function<void()>& bad : std::vector<std::function<void()>>{
+             [&] { Triangle t{o, x, Point3r{2, 0, 0}}; },
+             [&] { Triangle t{o, x, Point3r{0, nan, 0}}; },
+             [&] { Triangle t{o, x, y, {Point2r{0, 0}, Point2r{1, nan}, Point2r{1, 1}}}; },
+             [&] { TriangleMesh m{{o, x, Point3r{nan, 0, 0}}, {{0, 1, 2}}}; }})
+        try {
+            bad();
+        } catch (const std::invalid_argument&) {
+            ++triangle_errors;
+        }
+    Scene empty_mesh;
+    empty_mesh.add(TriangleMesh{{o, x, Point3r{2, 0, 0}}, {{0, 1, 2}}}, red);
+    const Triangle unit{o, x, y};
+    const bool triangle_ok = empty_mesh.shapeCount() == 0 && unit.normal().z == 1.0 &&
+                             unit.textureCoordinates()[2].x == 1.0 && unit.textureCoordinates()[2].y == 1.0;
+    std::printf("triangle_errors %d\ntriangle_ok %d\n", triangle_errors, triangle_ok ? 1 : 0);
+
     int errors_seen = 0;
     try {
         renderImage(RenderJob{camera, scene, 0, 7, PathEstimator{}, 1}, image);
@@ -186,5 +235,8 @@ int main(int argc, char** argv)
         ++errors_seen;
     }
     std::printf("errors %d\n", errors_seen);
-    return (errors_seen == 3 && sum > 0 && mismatches == 0 && samples_ok && size_errors == 3) ? 0 : 1;
+    return (errors_seen == 3 && sum > 0 && mismatches == 0 && samples_ok && size_errors == 3 && forms_equal &&
+            counts_ok && triangle_errors == 4 && triangle_ok)
+               ? 0
+               : 1;
 }

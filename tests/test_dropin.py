@@ -60,6 +60,9 @@ def test_dropin_render_runs_on_the_gpu(ctx, tmp_path):
     assert lines["errors"] == "3" and lines["size_errors"] == "3"
     assert lines["accumulate_mismatches"] == "0" and lines["add_sample_ok"] == "1"
     assert lines["default_is_scatter_only"] == "1"
+    # Triangle shapes, an added TriangleMesh and the identity instance: one image.
+    assert lines["triangle_forms_equal"] == "1" and lines["shape_counts_ok"] == "1"
+    assert lines["triangle_errors"] == "4" and lines["triangle_ok"] == "1"
     w, h = 64, 36
     img = np.fromfile(out, np.float32).reshape(h, w, 3)
     s = dropin_scene()
