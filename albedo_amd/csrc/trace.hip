@@ -81,9 +81,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(queryWav
                 if (ANY) {
                     occluded[id] = q.best.object != kMissId ? 1 : 0;
                 } else {
-                    const bool miss = q.best.object == kMissId;
-                    hits[id] = make_uint4(miss ? 0x7f800000u : __float_as_uint(q.best.t), q.best.object, q.best.prim,
-                                          miss ? 0u : packBary(q.best.b1, q.best.b2));
+                    const HitF h = q.result();
+                    const bool miss = h.object == kMissId;
+                    hits[id] = make_uint4(miss ? 0x7f800000u : __float_as_uint(h.t), h.object, h.prim,
+                                          miss ? 0u : packBary(h.b1, h.b2));
                 }
                 id = kNone;
             }

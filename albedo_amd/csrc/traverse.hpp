@@ -30,10 +30,10 @@ struct HitF {
 
 __device__ __forceinline__ bool betterHit(float t, uint32_t obj, uint32_t prim, const HitF& h)
 {
-    if (h.object == kMissId) return true;
-    if (t != h.t) return t < h.t;
-    if (obj != h.object) return obj < h.object;
-    return prim < h.prim;
+    // (t, object, prim) lexicographic order, any hit before none; written with
+    // bitwise operators so it compiles to compares without branches (a NaN t
+    // compares false both ways and never wins over a hit).
+    return (h.object == kMissId) | (t < h.t) | ((t == h.t) & ((obj < h.object) | ((obj == h.object) & (prim < h.prim))));
 }
 
 // Sphere test in FP64 (reference geometry/sphere.cpp:92-135): Haines'
@@ -459,7 +459,8 @@ struct RayQuery {
     RayF r;          // world-space ray
     BoxRay box;      // box-test form of the ray of the current level (world or instance space)
     ShearRay sh;     // triangle-test form of the instance-space ray
-    HitF best;
+    HitF best;       // b1, b2 hold Woop's v, w until result() divides them by bdet
+    float bdet;      // the best triangle's det (1 for a sphere or no hit)
     float lim;       // box-culling limit: cullLimit of best.t, updated with it
     int32_t ref;     // next node or leaf to visit; kNoChild when the lane must pop
     int32_t leaf;    // BLAS leaf whose triangles are being tested, or kNoChild
@@ -475,6 +476,7 @@ struct RayQuery {
         best.object = kMissId;
         best.prim = kMissId;
         best.b1 = best.b2 = 0.0f;
+        bdet = 1.0f;
         lim = cullLimit(r.tmax);
         box = makeBoxRay(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz);
         ref = WIDE ? sc.wide_root : sc.tlas_root;
@@ -493,7 +495,21 @@ struct RayQuery {
     {
         if (h.object == kMissId) return;
         best = h;
+        bdet = 1.0f;
         lim = cullLimit(h.t);
+    }
+
+    // The best hit with its barycentrics. Each accepted triangle keeps v, w and
+    // det; the two divisions run once here instead of at every acceptance, with
+    // the same operands, so the values are those an immediate division gives.
+    __device__ __forceinline__ HitF result() const
+    {
+        HitF h = best;
+        if (!ANY) {
+            h.b1 = best.b1 / bdet;
+            h.b2 = best.b2 / bdet;
+        }
+        return h;
     }
 
     __device__ __forceinline__ bool found() const { return ANY && best.object != kMissId; }
@@ -547,8 +563,9 @@ struct RayQuery {
                     best.t = t;
                     best.object = obj;
                     best.prim = prim;
-                    best.b1 = ANY ? 0.0f : v / det;
-                    best.b2 = ANY ? 0.0f : w / det;
+                    best.b1 = ANY ? 0.0f : v;
+                    best.b2 = ANY ? 0.0f : w;
+                    bdet = ANY ? 1.0f : det;
                     if (!ANY) lim = cullLimit(t);
                 }
             }
@@ -629,6 +646,7 @@ struct RayQuery {
                     best.prim = kMissId;
                     best.b1 = 0.0f;
                     best.b2 = 0.0f;
+                    bdet = 1.0f;
                     if (!ANY) lim = cullLimit(tf);
                 }
             }

@@ -1024,7 +1024,7 @@ __device__ __forceinline__ void finishQuery(const PathState& ps, const DevScene&
             ps.rad[slot] = rr;
         }
     } else {
-        const HitF& h = q.best;
+        const HitF h = q.result();
         ps.hit[slot] = make_uint4(__float_as_uint(h.t), h.object, h.prim, __float_as_uint(h.b1));
         ps.hit_b2[slot] = h.b2;
         ps.key[pos] = 1u << materialBin(sc, h.object);
