@@ -57,10 +57,11 @@ class Context:
     def check(self, status):
         check(status, self.handle)
 
-    def scene(self, scene: Scene, device_blas=False, wide_bvh=False) -> "GpuScene":
+    def scene(self, scene: Scene, device_blas=False, wide_bvh=False, one_pass=False) -> "GpuScene":
         """Uploads `scene`; device_blas builds its mesh BVHs on the GPU (AB_BUILD_DEVICE_BLAS),
-        wide_bvh traces ray queries over four-wide trees (AB_BUILD_WIDE_BVH)."""
-        return GpuScene(self, scene, device_blas, wide_bvh)
+        wide_bvh traces ray queries over four-wide trees (AB_BUILD_WIDE_BVH), one_pass
+        has the path tracer trace every object in one pass (AB_BUILD_ONE_PASS)."""
+        return GpuScene(self, scene, device_blas, wide_bvh, one_pass)
 
     def set_launch_config(self, block_size=256, paths_in_flight=1 << 21, grid_blocks=0, use_graph=1):
         c = _abi.LaunchConfig(block_size, paths_in_flight, grid_blocks, use_graph)
@@ -89,11 +90,12 @@ class Context:
 class GpuScene:
     """A scene resident in HBM (ab_scene_create)."""
 
-    def __init__(self, ctx: Context, scene: Scene, device_blas=False, wide_bvh=False):
+    def __init__(self, ctx: Context, scene: Scene, device_blas=False, wide_bvh=False, one_pass=False):
         self.ctx = ctx
         self.desc = scene.to_desc()
         self.desc.desc.build_flags = ((_abi.BUILD_DEVICE_BLAS if device_blas else 0) |
-                                      (_abi.BUILD_WIDE_BVH if wide_bvh else 0))
+                                      (_abi.BUILD_WIDE_BVH if wide_bvh else 0) |
+                                      (_abi.BUILD_ONE_PASS if one_pass else 0))
         h = ctypes.c_void_p()
         ctx.check(ctx.lib.ab_scene_create(ctx.handle, self.desc.ptr(), ctypes.byref(h)))
         self.handle = h

@@ -32,6 +32,7 @@ SCHED_FIFO, SCHED_SHARE, SCHED_SMALL_FIRST = 0, 1, 2
 JOB_KEEP_SUMS = 1
 BUILD_DEVICE_BLAS = 1
 BUILD_WIDE_BVH = 2
+BUILD_ONE_PASS = 4
 SUMS_FRACTION_BITS = 28
 
 AB_MISS = 0xFFFFFFFF

@@ -186,6 +186,14 @@ typedef struct ab_scene_desc {
  * builds a binary tree). */
 #define AB_BUILD_WIDE_BVH 2u
 
+/* Trace the scene in one pass. By default the path tracer traces a scene that
+ * holds both spheres and mesh instances in two passes: the spheres one ray per
+ * lane, then the instances with lane refill, starting from the sphere hit.
+ * Images are the same bit for bit either way; the flag is for comparing the
+ * two. No counterpart in the reference (scene/scene.cpp:107-133 traces one
+ * hierarchy). */
+#define AB_BUILD_ONE_PASS 4u
+
 /* Pinhole camera (reference CameraSettings + PerspectiveCamera,
  * camera/perspective_camera.hpp, perspective_camera.cpp:74-110). */
 typedef struct ab_camera {

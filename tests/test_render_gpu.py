@@ -166,6 +166,51 @@ def test_split_tracing_equals_single_scene_tracing(ctx, room):
         assert np.array_equal(single.view(np.uint32), out.view(np.uint32))
 
 
+def test_two_pass_tracing_equals_one_pass(ctx, room):
+    """Scenes that hold spheres and mesh instances are traced in two passes: the
+    spheres one ray per lane, then the instances with lane refill from the sphere
+    hit. Built with AB_BUILD_ONE_PASS they are traced in one. Images are
+    identical bit for bit, alone and in a batch (the batch traces the first pass
+    per ray's scene tag); a scene of meshes only has one pass either way."""
+    from albedo_amd.scene import FLAT, MeshGeometry, MeshInstance, Transform
+    _, g_room, _ = room
+    full = W.full_scene()
+    ico = W.icosphere_scene(level=3)
+    # Meshes only: one pass either way.
+    knots = Scene()
+    knots.add(MeshInstance(MeshGeometry(W.checker_floor(), FLAT), Transform()),
+              DiffuseMaterial(ConstantTexture((0.7, 0.7, 0.7))))
+    knot = MeshGeometry(W.trefoil_mesh(segments=48, sides=8), 1)
+    for k in range(3):
+        knots.add(MeshInstance(knot, Transform.translation((2.2 * k - 2.2, 1.2, -1.5))),
+                  DiffuseMaterial(ConstantTexture((0.8, 0.5 + 0.1 * k, 0.3))))
+    knots.add_light(PointLight((1.0, 5.0, 4.0), (40.0, 40.0, 40.0)))
+    knots.set_sky(SkyLight.uniform((0.1, 0.1, 0.12)))
+    small = Scene()
+    small.add(MeshInstance(MeshGeometry(W.checker_floor(), FLAT), Transform()), DiffuseMaterial(ConstantTexture((0.7, 0.7, 0.7))))
+    small.add(Sphere((0.0, 1.0, -1.0), 1.0), DiffuseMaterial(ConstantTexture((0.3, 0.5, 0.7))))
+    small.add_light(PointLight((2.0, 4.0, 3.0), (20.0, 20.0, 20.0)))
+    small.set_sky(SkyLight.uniform((0.1, 0.1, 0.12)))
+    cam = W.ROOM_CAMERA
+    room_cam = CameraSettings(cam.look_from, cam.look_at, (0, 1, 0), 38.0, 16 / 9)
+    cases = [(full, room_cam, 16), (ico, W.ICOSPHERE_CAMERA, 8), (knots, room_cam, 8), (small, room_cam, 8)]
+    two = [ctx.scene(s) for s, _, _ in cases]
+    one = [ctx.scene(s, one_pass=True) for s, _, _ in cases]
+    jobs = []
+    for k, ((s, c, depth), g2, g1) in enumerate(zip(cases, two, one)):
+        a = g2.render(c, 64, 36, 2, max_depth=depth, roulette_start=3, seed=20 + k)
+        b = g1.render(c, 64, 36, 2, max_depth=depth, roulette_start=3, seed=20 + k)
+        assert float(a.max()) > 0.0
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), k
+        jobs.append((g2, c, W.render_params(64, 36, 2, depth, 3, 20 + k)))
+    jobs.append((g_room, CameraSettings(cam.look_from, cam.look_at, (0, 1, 0), 38.0, 40 / 24),
+                 W.render_params(40, 24, 2, 8, 3, 4)))
+    outs, _ = batch(ctx, jobs)
+    for k, (g2, c, p) in enumerate(jobs[:-1]):
+        alone = one[k].render(c, p.width, p.height, p.samples_per_axis, p.max_depth, p.roulette_start, p.seed)
+        assert np.array_equal(alone.view(np.uint32), outs[k].view(np.uint32)), k
+
+
 @pytest.mark.parametrize("block,grid", [(128, 0), (64, 8)])
 def test_split_tracing_with_full_queues_traces_every_ray(ctx, room, block, grid):
     """Split tracing with the queue full: 16 384 paths in flight, so each wave of
