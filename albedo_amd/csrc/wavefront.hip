@@ -1009,6 +1009,22 @@ __global__ __launch_bounds__(BLOCK) void kShadow(PathState ps)
 constexpr uint32_t kQueueRefill = 16; // free lanes that trigger a refill
 constexpr uint32_t kQueueSteps = 16;  // traversal steps between refill checks
 
+// Up to kQueueSteps steps of the lanes holding a ray (called by those lanes
+// only). Without the FP64 sphere test (EARLY), the wave leaves early once
+// kQueueRefill lanes are free or done while its slice has rays left to start
+// (`more`, wave-uniform), so lanes whose rays end early (culled at the root, or
+// beyond the sphere hit) take new ones instead of waiting out the chunk. The
+// variants with the sphere test keep whole chunks: at their 80 VGPRs the check
+// makes the compiler spill.
+template <bool EARLY, class Query>
+__device__ __forceinline__ void stepQueue(Query& q, const DevScene& sc, LdsStack<64> stk, bool more)
+{
+    for (uint32_t k = 0; k < kQueueSteps && !q.done; ++k) {
+        q.step(sc, stk);
+        if (EARLY && more && uint32_t(__popcll(__ballot(!q.done))) <= 64 - kQueueRefill) break;
+    }
+}
+
 // Writes the finished query of the ray in `slot` at queue position `pos`.
 template <bool ANY, class Query>
 __device__ __forceinline__ void finishQuery(const PathState& ps, const DevScene& sc, const Query& q, uint32_t slot,
@@ -1148,7 +1164,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(refillWaves<
         // Done when no lane holds a ray and the slice is used up.
         if (__ballot(id != kNone) == 0 && next >= end) break;
         if (id != kNone) {
-            for (uint32_t k = 0; k < kQueueSteps && !q.done; ++k) q.step(sc, stk);
+            if constexpr (SPHERES) {
+                for (uint32_t k = 0; k < kQueueSteps && !q.done; ++k) q.step(sc, stk);
+            } else {
+                stepQueue<true>(q, sc, stk, next < end);
+            }
             if (q.done) {
                 finishQuery<ANY>(ps, sc, q, slot, id);
                 id = kNone;
@@ -1283,7 +1303,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(refillWaves<
         // past a window of other kernels' rays, so `next` reaches `end`.
         if (__ballot(id != kNone) == 0 && next >= end) break;
         if (id != kNone) {
-            for (uint32_t k = 0; k < kQueueSteps && !q.done; ++k) q.step(sc, stk);
+            if constexpr (SPHERES) {
+                for (uint32_t k = 0; k < kQueueSteps && !q.done; ++k) q.step(sc, stk);
+            } else {
+                stepQueue<true>(q, sc, stk, next < end);
+            }
             if (q.done) {
                 finishQuery<ANY>(ps, sc, q, slot, id);
                 id = kNone;
