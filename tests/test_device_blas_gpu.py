@@ -67,7 +67,7 @@ def test_full_scene_render_identical(ctx):
 def test_unknown_build_flag_rejected(ctx):
     s = W.room_scene()
     d = s.to_desc()
-    d.desc.build_flags = 6
+    d.desc.build_flags = 8  # the first bit no AB_BUILD_* flag uses
     h = ctypes.c_void_p()
     with pytest.raises(AlbedoError, match="build flags"):
         ctx.check(ctx.lib.ab_scene_create(ctx.handle, d.ptr(), ctypes.byref(h)))
