@@ -668,15 +668,26 @@ constexpr uint32_t kQueueActive = 1u, kQueueShadow = 2u, kQueueTerm = 4u;
 // Shades the path in `slot` and returns the queues it joins.
 __device__ __forceinline__ uint32_t shadePath(const PathState& ps, uint32_t slot)
 {
-        const float4 o4 = ps.org[slot], d4 = ps.dir[slot], t4 = ps.thr[slot], r4 = ps.rad[slot],
-                     p4 = ps.prev[slot];
+        const float4 o4 = ps.org[slot], d4 = ps.dir[slot], t4 = ps.thr[slot], p4 = ps.prev[slot];
         const uint32_t j = __float_as_uint(p4.w);
         const DevJob& job = ps.jobs[j];
         const DevScene& sc = job.scene;
         const F3 ro = f3(o4.x, o4.y, o4.z), rd = f3(d4.x, d4.y, d4.z), arr_o = f3(p4.x, p4.y, p4.z);
         const float arr_pdf = o4.w;
         const bool arr_spec = d4.w != 0.0f;
-        F3 thr = f3(t4.x, t4.y, t4.z), L = f3(r4.x, r4.y, r4.z);
+        F3 thr = f3(t4.x, t4.y, t4.z);
+        // The path's radiance, read and written only by the paths whose radiance
+        // changes here (most bounces add nothing); additions keep their order.
+        F3 L = f3(0.0f, 0.0f, 0.0f);
+        bool l_set = false;
+        auto addL = [&](F3 x) {
+            if (!l_set) {
+                const float4 r4 = ps.rad[slot];
+                L = f3(r4.x, r4.y, r4.z);
+                l_set = true;
+            }
+            L = L + x;
+        };
         const int depth = __float_as_int(t4.w);
         const uint4 h = ps.hit[slot];
         // Ids out of the scene's range (a hit that was never written for this
@@ -702,11 +713,12 @@ __device__ __forceinline__ uint32_t shadePath(const PathState& ps, uint32_t slot
                 // White straight down to (0.5, 0.7, 1) straight up, on the normalised direction.
                 const float up = 0.5f * (normalize(rd).y + 1.0f);
                 L = f3(1.0f, 1.0f, 1.0f) * (1.0f - up) + f3(0.5f, 0.7f, 1.0f) * up;
+                l_set = true;
             } else if (sc.sky_light >= 0) {
                 const DevLight& sky = sc.lights[sc.sky_light];
                 F3 e = skyRadiance(sky, rd);
                 if (mis && !isBlack(e) && !arr_spec) e = e * powerHeuristic(arr_pdf, (1.0f / nl) * kUniformSpherePdf);
-                L = L + thr * e;
+                addL(thr * e);
             }
         } else {
             const DevObject& obj = sc.objects[h.y];
@@ -715,13 +727,14 @@ __device__ __forceinline__ uint32_t shadePath(const PathState& ps, uint32_t slot
             const SurfaceHit s = surfaceAt(sc, h, ps.hit_b2[slot], ro, rd, uv_needed);
             if (est == AB_ESTIMATOR_NORMALS) {
                 L = (s.ns + f3(1.0f, 1.0f, 1.0f)) * 0.5f;
+                l_set = true;
             } else if (obj.light >= 0 && s.front) {
                 const DevLight& l = sc.lights[obj.light];
                 if (l.kind == 1u) {
                     F3 e = f3(l.a[0], l.a[1], l.a[2]);
                     if (mis && !isBlack(e) && !arr_spec)
                         e = e * powerHeuristic(arr_pdf, (1.0f / nl) * lightPdf(l, arr_o, rd));
-                    L = L + thr * e;
+                    addL(thr * e);
                 }
             }
             if (est != AB_ESTIMATOR_NORMALS && depth < int(job.max_depth)) {
@@ -768,7 +781,7 @@ __device__ __forceinline__ uint32_t shadePath(const PathState& ps, uint32_t slot
                                 ps.sh_c[slot] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
                                 shadow = true;
                             } else {
-                                L = L + contrib;
+                                addL(contrib);
                             }
                         }
                     }
@@ -796,7 +809,7 @@ __device__ __forceinline__ uint32_t shadePath(const PathState& ps, uint32_t slot
                 }
             }
         }
-        ps.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
+        if (l_set) ps.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
         return (alive ? kQueueActive : kQueueTerm) | (shadow ? kQueueShadow : 0u);
 }
 
