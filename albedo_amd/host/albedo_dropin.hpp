@@ -11,7 +11,9 @@
 //   Diffuse/Conductor/Dielectric/MixMaterial, PerlinPattern  material/*.hpp
 //   PointLight, SkyLight           light/point_light.hpp, light/sky_light.hpp
 //   Scene                          scene/scene.hpp:31-98 (add of a shape or a TriangleMesh,
-//                                  addAreaLight, addLight, setSky, shapeCount)
+//                                  addAreaLight, addLight, setSky, sky, shapeCount,
+//                                  intersect, occluded)
+//   Ray, Interval, HitRecord       core/ray.hpp, core/interval.hpp, scene/hit_record.hpp
 //   CameraSettings, PerspectiveCamera  camera/perspective_camera.hpp:16-53
 //   PathEstimator                  estimator/path_estimator.hpp:22-38
 //   RenderJob, renderImage         render/render_image.hpp:17-49
@@ -29,6 +31,7 @@
 #include <cstdint>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -56,6 +59,35 @@ struct Point2i {
 };
 struct Normal3r {
     Real x{}, y{}, z{};
+};
+
+// A ray and a parameter range (reference core/ray.hpp:13-19, core/interval.hpp:15-40).
+struct Ray {
+    Point3r origin{};
+    Vector3r direction{};
+    Point3r at(Real t) const noexcept
+    {
+        return {origin.x + direction.x * t, origin.y + direction.y * t, origin.z + direction.z * t};
+    }
+};
+struct Interval {
+    Real min{std::numeric_limits<Real>::infinity()};
+    Real max{-std::numeric_limits<Real>::infinity()};
+    bool isEmpty() const noexcept { return min > max; }
+    bool contains(Real x) const noexcept { return min <= x && x <= max; }
+    bool surrounds(Real x) const noexcept { return min < x && x < max; }
+};
+
+// The nearest surface a ray hits (reference scene/hit_record.hpp:13-33): normals
+// face against the ray, `object` counts objects as the reference adds them.
+struct HitRecord {
+    Real t{};
+    Point3r point{};
+    Normal3r normal{};
+    Normal3r shading_normal{};
+    bool front_face{};
+    Point2r uv{};
+    std::size_t object{};
 };
 struct Rgb {
     Real r{}, g{}, b{};
@@ -187,6 +219,22 @@ public:
         return {v[0] + off_[0], v[1] + off_[1], v[2] + off_[2]};
     }
 
+    Vector3r apply(Vector3r d) const
+    {
+        const auto v = detail::apply(lin_, {d.x, d.y, d.z});
+        return {v[0], v[1], v[2]};
+    }
+
+    // Normals map by the transpose of the inverse (reference core/transform.cpp:202-205).
+    Normal3r apply(Normal3r n) const
+    {
+        Real r[3];
+        for (int k = 0; k < 3; ++k) r[k] = ilin_[0][k] * n.x + ilin_[1][k] * n.y + ilin_[2][k] * n.z;
+        return {r[0], r[1], r[2]};
+    }
+
+    Ray apply(const Ray& ray) const { return {apply(ray.origin), apply(ray.direction)}; }
+
     // Arvo's transformed box (reference core/transform.cpp:118-132, 212-225).
     void applyBox(const Point3r& lo, const Point3r& hi, Point3r& out_lo, Point3r& out_hi) const
     {
@@ -221,6 +269,113 @@ private:
     detail::Mat3 lin_{detail::identity()}, ilin_{detail::identity()};
     std::array<Real, 3> off_{}, ioff_{};
 };
+
+namespace detail {
+
+using V3 = std::array<Real, 3>;
+inline V3 sub(V3 a, V3 b) { return {a[0] - b[0], a[1] - b[1], a[2] - b[2]}; }
+inline V3 cross(V3 a, V3 b)
+{
+    return {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+}
+inline Real dot(V3 a, V3 b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline V3 v3(Point3r p) { return {p.x, p.y, p.z}; }
+inline V3 v3(Vector3r p) { return {p.x, p.y, p.z}; }
+
+// One primitive's hit in FP64, as the reference's shapes report it: t, point,
+// outward geometric and shading normals (unit), texture coordinates.
+struct PrimHit {
+    Real t{};
+    V3 point{}, normal{}, shading{};
+    Point2r uv{};
+};
+
+// Haines' sphere test with the stable roots (reference geometry/sphere.cpp:66-135)
+// and its texture coordinates (:137-147).
+inline std::optional<PrimHit> sphereHit(Point3r c, Real r, const Ray& ray, Interval range)
+{
+    const V3 d = v3(ray.direction), l = sub(v3(ray.origin), v3(c));
+    const Real a = dot(d, d);
+    if (!std::isnormal(a)) return std::nullopt;
+    const Real h = dot(d, l), cc = dot(l, l) - r * r;
+    const V3 off = {l[0] - d[0] * (h / a), l[1] - d[1] * (h / a), l[2] - d[2] * (h / a)};
+    const Real disc = a * (r * r - dot(off, off));
+    if (disc < 0) return std::nullopt;
+    const Real q = -(h + std::copysign(std::sqrt(disc), h));
+    Real t0 = 0, t1 = 0;
+    if (q != 0) {
+        t0 = q / a;
+        t1 = cc / q;
+        if (t0 > t1) std::swap(t0, t1);
+    }
+    Real t = t0;
+    if (!range.surrounds(t)) {
+        t = t1;
+        if (!range.surrounds(t)) return std::nullopt;
+    }
+    PrimHit hit;
+    hit.t = t;
+    const Point3r p = ray.at(t);
+    hit.point = v3(p);
+    const V3 out = sub(hit.point, v3(c));
+    const Real l2 = dot(out, out);
+    if (!(l2 > 0)) return std::nullopt;
+    const Real il = 1 / std::sqrt(l2);
+    hit.normal = {out[0] * il, out[1] * il, out[2] * il};
+    hit.shading = hit.normal;
+    const Real pi = 3.14159265358979323846;
+    const Real lon = std::atan2(-hit.normal[2], hit.normal[0]) + pi;
+    const Real lat = std::acos(std::clamp(-hit.normal[1], Real(-1), Real(1)));
+    hit.uv = {lon / (2 * pi), lat / pi};
+    return hit;
+}
+
+// Woop, Benthin and Wald's watertight test in FP64 (reference
+// geometry/triangle.cpp:95-176): the ray moved to the origin and sheared onto +z;
+// the point is interpolated from the corners, which keeps it on the plane.
+inline std::optional<PrimHit> triangleHit(const std::array<V3, 3>& p, const std::array<Point2r, 3>& uv,
+                                          const Ray& ray, Interval range)
+{
+    const V3 d = v3(ray.direction), o = v3(ray.origin);
+    if (!std::isnormal(dot(d, d))) return std::nullopt;
+    const V3 m = {std::abs(d[0]), std::abs(d[1]), std::abs(d[2])};
+    const int kz = m[0] > m[1] ? (m[0] > m[2] ? 0 : 2) : (m[1] > m[2] ? 1 : 2);
+    int kx = (kz + 1) % 3, ky = (kx + 1) % 3;
+    if (d[kz] < 0) std::swap(kx, ky);
+    const Real sx = d[kx] / d[kz], sy = d[ky] / d[kz], sz = 1 / d[kz];
+    V3 q[3];
+    for (int i = 0; i < 3; ++i) {
+        const V3 off = sub(p[i], o);
+        q[i] = {off[kx] - sx * off[kz], off[ky] - sy * off[kz], sz * off[kz]};
+    }
+    auto edge = [](const V3& a, const V3& b) { return b[0] * a[1] - b[1] * a[0]; };
+    const Real u = edge(q[1], q[2]), v = edge(q[2], q[0]), w = edge(q[0], q[1]);
+    const bool inside = (u >= 0 && v >= 0 && w >= 0) || (u <= 0 && v <= 0 && w <= 0);
+    if (!inside) return std::nullopt;
+    const Real det = u + v + w;
+    if (det == 0) return std::nullopt;
+    const Real t = (u * q[0][2] + v * q[1][2] + w * q[2][2]) / det;
+    if (!range.surrounds(t)) return std::nullopt;
+    const Real b0 = u / det, b1 = v / det, b2 = w / det;
+    PrimHit hit;
+    hit.t = t;
+    const V3 e1 = sub(p[1], p[0]), e2 = sub(p[2], p[0]);
+    for (int k = 0; k < 3; ++k) hit.point[k] = p[0][k] + e1[k] * b1 + e2[k] * b2;
+    const V3 n = cross(e1, e2);
+    const Real il = 1 / std::sqrt(dot(n, n));
+    hit.normal = {n[0] * il, n[1] * il, n[2] * il};
+    hit.shading = hit.normal;
+    hit.uv = {b0 * uv[0].x + b1 * uv[1].x + b2 * uv[2].x, b0 * uv[0].y + b1 * uv[1].y + b2 * uv[2].y};
+    return hit;
+}
+
+inline const std::array<Point2r, 3>& defaultUv()
+{
+    static const std::array<Point2r, 3> uv = {Point2r{0, 0}, Point2r{1, 0}, Point2r{1, 1}};
+    return uv;
+}
+
+} // namespace detail
 
 class Sphere {
 public:
@@ -365,6 +520,48 @@ public:
                     uv_.push_back(float(mesh.textureCoordinates()[i].y));
                 }
         }
+        // Hit records (Scene::intersect): the faces in the order the device numbers
+        // them — those whose corners, rounded to the FP32 the device stores, still
+        // span an area, as the device's build decides — with FP64 corners, corner
+        // texture coordinates and, for smooth shading, area-weighted unit corner
+        // normals (reference triangle_mesh.cpp:92-125, over the FP64 positions).
+        std::vector<detail::V3> sums(shading == MeshShading::Smooth ? P.size() : 0, detail::V3{0, 0, 0});
+        for (std::size_t fi = 0; fi < mesh.faces().size(); ++fi) {
+            const auto& f = mesh.faces()[fi];
+            detail::V3 fp[3];
+            for (int k = 0; k < 3; ++k) fp[k] = {double(float(P[f[k]].x)), double(float(P[f[k]].y)), double(float(P[f[k]].z))};
+            const detail::V3 fn = detail::cross(detail::sub(fp[1], fp[0]), detail::sub(fp[2], fp[0]));
+            if (std::isnormal(detail::dot(fn, fn))) kept_.push_back(uint32_t(fi));
+            if (!sums.empty() && formsTriangle(P[f[0]], P[f[1]], P[f[2]])) {
+                const detail::V3 an = detail::cross(detail::sub(detail::v3(P[f[1]]), detail::v3(P[f[0]])),
+                                                    detail::sub(detail::v3(P[f[2]]), detail::v3(P[f[0]])));
+                for (std::size_t i : f)
+                    for (int k = 0; k < 3; ++k) sums[i][k] += an[k];
+            }
+        }
+        for (uint32_t fi : kept_) {
+            const auto& f = mesh.faces()[fi];
+            Face face;
+            for (int k = 0; k < 3; ++k) face.p[k] = detail::v3(P[f[k]]);
+            face.uv = detail::defaultUv();
+            if (!mesh.textureFaces().empty())
+                for (int k = 0; k < 3; ++k) face.uv[k] = mesh.textureCoordinates()[mesh.textureFaces()[fi][k]];
+            if (!sums.empty()) {
+                const detail::V3 fn = detail::cross(detail::sub(face.p[1], face.p[0]), detail::sub(face.p[2], face.p[0]));
+                const Real fl2 = detail::dot(fn, fn);
+                const detail::V3 face_n = std::isnormal(fl2) ? detail::V3{fn[0] / std::sqrt(fl2), fn[1] / std::sqrt(fl2),
+                                                                          fn[2] / std::sqrt(fl2)}
+                                                             : detail::V3{0, 0, 0};
+                for (int k = 0; k < 3; ++k) {
+                    const detail::V3& sm = sums[f[k]];
+                    const Real l2 = detail::dot(sm, sm);
+                    face.n[k] = std::isnormal(l2) ? detail::V3{sm[0] / std::sqrt(l2), sm[1] / std::sqrt(l2),
+                                                               sm[2] / std::sqrt(l2)}
+                                                  : face_n;
+                }
+            }
+            faces_.push_back(face);
+        }
     }
     MeshShading shading() const { return shading_; }
     Point3r boundsMin() const { return lo_; }
@@ -373,14 +570,67 @@ public:
     const std::vector<float>& positions() const { return pos_; }
     const std::vector<uint32_t>& indices() const { return idx_; }
     const std::vector<float>& uv() const { return uv_; }
+    // Faces with area in the device's numbering (hit records' prim ids).
+    std::size_t keptCount() const { return faces_.size(); }
+
+    // The FP64 hit on kept face `k` of a ray in mesh space (reference
+    // geometry/mesh_geometry.cpp:77-98, with the smooth normal blended at the
+    // point's barycentric coordinates, :53-64). `t_device` is the device's t for
+    // a hit the FP64 test misses by rounding (at an edge): the point then lies at
+    // that t, with the face's normal.
+    detail::PrimHit hitFace(std::size_t k, const Ray& ray, Interval range, Real t_device) const
+    {
+        const Face& f = faces_[k];
+        const std::array<detail::V3, 3> p = {f.p[0], f.p[1], f.p[2]};
+        std::optional<detail::PrimHit> h = detail::triangleHit(p, f.uv, ray, range);
+        if (!h) {
+            detail::PrimHit g;
+            g.t = t_device;
+            g.point = detail::v3(ray.at(t_device));
+            const detail::V3 n = detail::cross(detail::sub(p[1], p[0]), detail::sub(p[2], p[0]));
+            const Real il = 1 / std::sqrt(detail::dot(n, n));
+            g.normal = g.shading = {n[0] * il, n[1] * il, n[2] * il};
+            const auto b = barycentric(p, g.point);
+            g.uv = {b[0] * f.uv[0].x + b[1] * f.uv[1].x + b[2] * f.uv[2].x,
+                    b[0] * f.uv[0].y + b[1] * f.uv[1].y + b[2] * f.uv[2].y};
+            h = g;
+        }
+        if (shading_ == MeshShading::Smooth) {
+            const auto b = barycentric(p, h->point);
+            detail::V3 bl{};
+            for (int c = 0; c < 3; ++c)
+                for (int q = 0; q < 3; ++q) bl[q] += f.n[c][q] * b[c];
+            const Real l2 = detail::dot(bl, bl);
+            if (l2 >= 1e-12) h->shading = {bl[0] / std::sqrt(l2), bl[1] / std::sqrt(l2), bl[2] / std::sqrt(l2)};
+        }
+        return *h;
+    }
 
 private:
+    struct Face {
+        detail::V3 p[3];
+        std::array<Point2r, 3> uv;
+        detail::V3 n[3]{};
+    };
+
+    // Signed-area weights of `x` (reference geometry/triangle.cpp:183-196).
+    static std::array<Real, 3> barycentric(const std::array<detail::V3, 3>& p, const detail::V3& x)
+    {
+        const detail::V3 e1 = detail::sub(p[1], p[0]), e2 = detail::sub(p[2], p[0]), off = detail::sub(x, p[0]);
+        const detail::V3 an = detail::cross(e1, e2);
+        const Real a2 = detail::dot(an, an);
+        const Real b1 = detail::dot(detail::cross(off, e2), an) / a2, b2 = detail::dot(detail::cross(e1, off), an) / a2;
+        return {1 - b1 - b2, b1, b2};
+    }
+
     MeshShading shading_;
     Point3r lo_{}, hi_{};
     std::vector<Point3r> used_;
     std::vector<float> pos_;
     std::vector<uint32_t> idx_;
     std::vector<float> uv_;
+    std::vector<uint32_t> kept_;
+    std::vector<Face> faces_;
 };
 
 class MeshInstance {
@@ -455,7 +705,7 @@ struct SkyLight {
 // Objects and lights; flattens into an ab_scene_desc when rendered.
 class Scene {
 public:
-    void add(const Shape& shape, const Material& material) { objects_.push_back({shape, material, -1, 1}); }
+    void add(const Shape& shape, const Material& material) { objects_.push_back({shape, material, -1, 1, false}); }
 
     // Every triangle of `mesh` with a computable area, in face order, with one
     // material (reference scene/scene.cpp:33-60). They become one flat mesh under
@@ -470,7 +720,7 @@ public:
         for (const auto& f : mesh.faces()) kept += formsTriangle(P[f[0]], P[f[1]], P[f[2]]) ? 1 : 0;
         if (kept == 0) return;
         auto geometry = std::make_shared<const MeshGeometry>(mesh, MeshShading::Flat);
-        objects_.push_back({MeshInstance{std::move(geometry), Transform{}}, material, -1, kept});
+        objects_.push_back({MeshInstance{std::move(geometry), Transform{}}, material, -1, kept, true});
     }
 
     void addAreaLight(const Sphere& sphere, Rgb radiance, const Material& material = DiffuseMaterial{})
@@ -481,7 +731,7 @@ public:
         l.object = uint32_t(objects_.size());
         l.radiance = radiance;
         lights_.push_back(l);
-        objects_.push_back({sphere, material, light, 1});
+        objects_.push_back({sphere, material, light, 1, false});
     }
 
     std::size_t addLight(const PointLight& light)
@@ -515,6 +765,54 @@ public:
         return n;
     }
 
+    // The light arriving from beyond the scene; black until set (reference
+    // scene/scene.hpp:57-58).
+    SkyLight sky() const { return sky_ ? lights_[*sky_].sky : SkyLight{}; }
+
+    // The nearest hit with t strictly inside `t_range` (reference
+    // scene/scene.cpp:107-133): the device picks the object and primitive (FP32
+    // traversal, the (t, object, prim) lexicographic minimum), then the record is
+    // computed in FP64 on that primitive as the reference's shapes do: t, point,
+    // normals facing against the ray, texture coordinates, and the object index
+    // as the reference numbers objects (each face of an added TriangleMesh one).
+    // One device query per call; the vector form below batches many rays.
+    std::optional<HitRecord> intersect(const Ray& ray, Interval t_range) const
+    {
+        return intersect(std::vector<Ray>{ray}, t_range)[0];
+    }
+
+    // True when any object lies along `ray` with t strictly inside `t_range`
+    // (reference scene/scene.cpp:157-165).
+    bool occluded(const Ray& ray, Interval t_range) const { return occluded(std::vector<Ray>{ray}, t_range)[0] != 0; }
+
+    // Many rays in one device query (no reference counterpart: the reference
+    // intersects one ray per call).
+    std::vector<std::optional<HitRecord>> intersect(const std::vector<Ray>& rays, Interval t_range) const
+    {
+        std::vector<std::optional<HitRecord>> out(rays.size());
+        if (rays.empty() || !(t_range.min < t_range.max)) return out;
+        const std::vector<ab_ray> r = deviceRays(rays, t_range);
+        std::vector<ab_hit> h(r.size());
+        detail::check(ab_trace_closest_host(device(), r.data(), h.data(), r.size()));
+        std::vector<std::size_t> base(objects_.size());
+        for (std::size_t k = 0, b = 0; k < objects_.size(); b += objects_[k].count, ++k) base[k] = b;
+        for (std::size_t i = 0; i < rays.size(); ++i) {
+            if (h[i].object == AB_MISS) continue;
+            out[i] = record(rays[i], t_range, h[i], base);
+        }
+        return out;
+    }
+
+    // Occlusion of many rays in one device query: 1 where something lies along the ray.
+    std::vector<uint8_t> occluded(const std::vector<Ray>& rays, Interval t_range) const
+    {
+        std::vector<uint8_t> out(rays.size(), 0);
+        if (rays.empty() || !(t_range.min < t_range.max)) return out;
+        const std::vector<ab_ray> r = deviceRays(rays, t_range);
+        detail::check(ab_trace_any_host(device(), r.data(), out.data(), r.size()));
+        return out;
+    }
+
     // The resident device scene (built on first use, shared by content hash).
     ab_scene* device() const
     {
@@ -531,7 +829,8 @@ private:
         Shape shape;
         Material material;
         int light;
-        std::size_t count;
+        std::size_t count; // objects the reference counts for it
+        bool faces;        // an added TriangleMesh: each kept face is an object of its own
     };
     struct Light {
         uint32_t kind{};
@@ -540,6 +839,82 @@ private:
         PointLight point{};
         SkyLight sky{};
     };
+
+    static std::vector<ab_ray> deviceRays(const std::vector<Ray>& rays, Interval t_range)
+    {
+        std::vector<ab_ray> r(rays.size());
+        for (std::size_t i = 0; i < rays.size(); ++i) {
+            const Ray& q = rays[i];
+            r[i] = ab_ray{{float(q.origin.x), float(q.origin.y), float(q.origin.z)},
+                          float(t_range.min),
+                          {float(q.direction.x), float(q.direction.y), float(q.direction.z)},
+                          float(t_range.max)};
+        }
+        return r;
+    }
+
+    // The FP64 record of the device's hit `h` (reference scene/scene.cpp:124-133).
+    HitRecord record(const Ray& ray, Interval range, const ab_hit& h, const std::vector<std::size_t>& base) const
+    {
+        const Object& o = objects_[h.object];
+        detail::PrimHit p;
+        std::size_t index = base[h.object];
+        if (const auto* s = std::get_if<Sphere>(&o.shape)) {
+            std::optional<detail::PrimHit> q = detail::sphereHit(s->center(), s->radius(), ray, range);
+            if (!q) {
+                // Rounding put the FP32 hit just inside the range and the FP64 roots
+                // just outside: the point at the device's t, the sphere's normal there.
+                detail::PrimHit g;
+                g.t = h.t;
+                g.point = detail::v3(ray.at(h.t));
+                const detail::V3 out = detail::sub(g.point, detail::v3(s->center()));
+                const Real il = 1 / std::sqrt(detail::dot(out, out));
+                g.normal = g.shading = {out[0] * il, out[1] * il, out[2] * il};
+                const Real pi = 3.14159265358979323846;
+                g.uv = {(std::atan2(-g.normal[2], g.normal[0]) + pi) / (2 * pi),
+                        std::acos(std::clamp(-g.normal[1], Real(-1), Real(1))) / pi};
+                q = g;
+            }
+            p = *q;
+        } else if (const auto* t = std::get_if<Triangle>(&o.shape)) {
+            const std::array<detail::V3, 3> c = {detail::v3(t->vertices()[0]), detail::v3(t->vertices()[1]),
+                                                 detail::v3(t->vertices()[2])};
+            std::optional<detail::PrimHit> q = detail::triangleHit(c, t->textureCoordinates(), ray, range);
+            p = q ? *q : MeshGeometry(TriangleMesh{{t->vertices()[0], t->vertices()[1], t->vertices()[2]},
+                                                   {{0, 1, 2}},
+                                                   {t->textureCoordinates()[0], t->textureCoordinates()[1],
+                                                    t->textureCoordinates()[2]},
+                                                   {{0, 1, 2}}})
+                             .hitFace(0, ray, range, h.t);
+        } else {
+            const auto& inst = std::get<MeshInstance>(o.shape);
+            if (o.faces) index += h.prim;
+            // Mesh space with the inverse map, t shared; normals back by the
+            // transpose of the inverse (reference geometry/mesh_instance.cpp:32-60).
+            const Transform to_object = inst.transform().inverse();
+            const Ray local = to_object.apply(ray);
+            p = inst.mesh().hitFace(h.prim, local, range, h.t);
+            const Point3r wp = inst.transform().apply(Point3r{p.point[0], p.point[1], p.point[2]});
+            p.point = detail::v3(wp);
+            const Normal3r wn = inst.transform().apply(Normal3r{p.normal[0], p.normal[1], p.normal[2]});
+            const Normal3r ws = inst.transform().apply(Normal3r{p.shading[0], p.shading[1], p.shading[2]});
+            const detail::V3 n = {wn.x, wn.y, wn.z}, sn = {ws.x, ws.y, ws.z};
+            const Real nl = std::sqrt(detail::dot(n, n)), sl = std::sqrt(detail::dot(sn, sn));
+            const bool flat = p.normal == p.shading;
+            p.normal = {n[0] / nl, n[1] / nl, n[2] / nl};
+            p.shading = flat || !std::isnormal(sl * sl) ? p.normal : detail::V3{sn[0] / sl, sn[1] / sl, sn[2] / sl};
+        }
+        HitRecord r;
+        r.t = p.t;
+        r.point = {p.point[0], p.point[1], p.point[2]};
+        r.front_face = detail::dot(detail::v3(ray.direction), p.normal) < 0;
+        const Real sign = r.front_face ? 1 : -1;
+        r.normal = {sign * p.normal[0], sign * p.normal[1], sign * p.normal[2]};
+        r.shading_normal = {sign * p.shading[0], sign * p.shading[1], sign * p.shading[2]};
+        r.uv = p.uv;
+        r.object = index;
+        return r;
+    }
 
     static void putRgb(float* d, Rgb c)
     {
