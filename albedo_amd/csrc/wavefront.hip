@@ -1023,18 +1023,17 @@ constexpr uint32_t kQueueRefill = 16; // free lanes that trigger a refill
 constexpr uint32_t kQueueSteps = 16;  // traversal steps between refill checks
 
 // Up to kQueueSteps steps of the lanes holding a ray (called by those lanes
-// only). Without the FP64 sphere test (EARLY), the wave leaves early once
-// kQueueRefill lanes are free or done while its slice has rays left to start
-// (`more`, wave-uniform), so lanes whose rays end early (culled at the root, or
-// beyond the sphere hit) take new ones instead of waiting out the chunk. The
-// variants with the sphere test keep whole chunks: at their 80 VGPRs the check
-// makes the compiler spill.
-template <bool EARLY, class Query>
+// only). The wave leaves early once kQueueRefill lanes are free or done while its
+// slice has rays left to start (`more`, wave-uniform), so lanes whose rays end
+// early (culled at the root, or beyond the sphere hit) take new ones instead of
+// waiting out the chunk. The refill kernels' variants with the FP64 sphere test
+// run whole chunks instead: at their 80 VGPRs this check makes the compiler spill.
+template <class Query>
 __device__ __forceinline__ void stepQueue(Query& q, const DevScene& sc, LdsStack<64> stk, bool more)
 {
     for (uint32_t k = 0; k < kQueueSteps && !q.done; ++k) {
         q.step(sc, stk);
-        if (EARLY && more && uint32_t(__popcll(__ballot(!q.done))) <= 64 - kQueueRefill) break;
+        if (more && uint32_t(__popcll(__ballot(!q.done))) <= 64 - kQueueRefill) break;
     }
 }
 
@@ -1180,7 +1179,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(refillWaves<
             if constexpr (SPHERES) {
                 for (uint32_t k = 0; k < kQueueSteps && !q.done; ++k) q.step(sc, stk);
             } else {
-                stepQueue<true>(q, sc, stk, next < end);
+                stepQueue(q, sc, stk, next < end);
             }
             if (q.done) {
                 finishQuery<ANY>(ps, sc, q, slot, id);
@@ -1319,7 +1318,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(refillWaves<
             if constexpr (SPHERES) {
                 for (uint32_t k = 0; k < kQueueSteps && !q.done; ++k) q.step(sc, stk);
             } else {
-                stepQueue<true>(q, sc, stk, next < end);
+                stepQueue(q, sc, stk, next < end);
             }
             if (q.done) {
                 finishQuery<ANY>(ps, sc, q, slot, id);
