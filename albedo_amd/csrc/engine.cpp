@@ -133,7 +133,8 @@ void chooseTracing(LaunchShape& s, const SceneSet& set, uint32_t capacity, bool 
 
 bool sameTracing(const LaunchShape& a, const LaunchShape& b)
 {
-    if (a.refill != b.refill || a.pstack != b.pstack || a.block != b.block) return false;
+    if (a.refill != b.refill || a.pstack != b.pstack || a.block != b.block || a.free_sort != b.free_sort)
+        return false;
     if (a.refill == kTracePlain) return true;
     return a.qgrid == b.qgrid && a.q_spheres == b.q_spheres && a.q_reg == b.q_reg && a.qstack == b.qstack &&
            a.sphere_pass == b.sphere_pass && a.sstack == b.sstack &&
@@ -528,6 +529,9 @@ ab_status Engine::renderBatch(const std::vector<JobSpec>& specs, const std::vect
     SceneSet set;
     for (const DevJob& j : jobs_) set.add(j.scene);
     chooseTracing(shape, set, capacity, true, cfg);
+    // Every sample starts in the first bounce when they all fit the paths in
+    // flight (the free stack starts sorted), so no bounce sorts the free slots.
+    shape.free_sort = total > capacity ? 1u : 0u;
     // Split tracing: the jobs of each scene with meshes share a scene tag, and
     // the one-ray-per-lane kernels take the stack class of the untagged scenes.
     clearTags();

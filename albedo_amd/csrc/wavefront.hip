@@ -1541,8 +1541,10 @@ hipError_t bounce(const PathState& ps, const LaunchShape& sh, hipStream_t s)
 {
     const uint32_t grid = sh.grid;
     const bool split = sh.refill == kTraceSplit;
-    hipLaunchKernelGGL(kFreeCount, dim3(kFreeGrid), dim3(256), 0, s, ps);
-    hipLaunchKernelGGL(kFreeScatter, dim3(kFreeGrid), dim3(256), 0, s, ps);
+    if (sh.free_sort) {
+        hipLaunchKernelGGL(kFreeCount, dim3(kFreeGrid), dim3(256), 0, s, ps);
+        hipLaunchKernelGGL(kFreeScatter, dim3(kFreeGrid), dim3(256), 0, s, ps);
+    }
     hipLaunchKernelGGL(kGenerate, dim3(grid), dim3(BLOCK), 0, s, ps);
     if (split) {
         hipLaunchKernelGGL(kTagCount, dim3(kTagGrid), dim3(kTagBlock), 0, s, ps);

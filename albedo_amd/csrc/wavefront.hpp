@@ -167,6 +167,10 @@ struct LaunchShape {
     uint32_t sphere_pass{0};
     uint32_t sstack{0};
     DevScene sphere_scene{};
+    // 0 when no bounce can append new samples after the first (all of a render's
+    // samples fit the paths in flight): the free-slot sort is left out, as it
+    // would find nothing to do.
+    uint32_t free_sort{1};
 };
 
 // Enqueues two bounces (generate, extend, bin, shade, shadow, finish, with the
