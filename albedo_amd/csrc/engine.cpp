@@ -133,7 +133,8 @@ void chooseTracing(LaunchShape& s, const SceneSet& set, uint32_t capacity, bool 
 
 bool sameTracing(const LaunchShape& a, const LaunchShape& b)
 {
-    if (a.refill != b.refill || a.pstack != b.pstack || a.block != b.block || a.free_sort != b.free_sort)
+    if (a.refill != b.refill || a.pstack != b.pstack || a.block != b.block || a.free_sort != b.free_sort ||
+        a.generate != b.generate)
         return false;
     if (a.refill == kTracePlain) return true;
     return a.qgrid == b.qgrid && a.q_spheres == b.q_spheres && a.q_reg == b.q_reg && a.qstack == b.qstack &&
@@ -444,7 +445,13 @@ uint32_t firstCheckPairs(const DevJob* jobs, size_t n_jobs)
 ab_status Engine::runUntilDone(const LaunchShape& shape, bool use_graph, Counters& c)
 {
     use_graph_ = use_graph;
-    const ab_status cst = capture(shape);
+    // When every sample starts in the first bounce (no free-slot sort), the
+    // bounces after the first pair generate nothing: that pair is launched
+    // directly and the pairs after it leave the generation kernel out.
+    LaunchShape steady = shape;
+    bool first_pair = shape.free_sort == 0;
+    if (first_pair) steady.generate = 0;
+    const ab_status cst = capture(steady);
     if (cst != AB_OK) return cst;
     // Bounces between checks grow geometrically: the queue drains in a tail of
     // short bounces, and each check costs a host round trip.
@@ -455,7 +462,12 @@ ab_status Engine::runUntilDone(const LaunchShape& shape, bool use_graph, Counter
         if (bounces > limit) return fail(AB_DEVICE_ERROR, "render did not finish within its bounce bound");
         bounces += 2 * pairs_per_check;
         for (uint32_t k = 0; k < pairs_per_check; ++k) {
-            const ab_status st = bouncePair(shape);
+            if (first_pair) {
+                HIP_TRY(launchBouncePair(ps_, shape, stream_), "bounce launch");
+                first_pair = false;
+                continue;
+            }
+            const ab_status st = bouncePair(steady);
             if (st != AB_OK) return st;
         }
         const ab_status st = readCounters(c);

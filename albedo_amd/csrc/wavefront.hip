@@ -1545,7 +1545,7 @@ hipError_t bounce(const PathState& ps, const LaunchShape& sh, hipStream_t s)
         hipLaunchKernelGGL(kFreeCount, dim3(kFreeGrid), dim3(256), 0, s, ps);
         hipLaunchKernelGGL(kFreeScatter, dim3(kFreeGrid), dim3(256), 0, s, ps);
     }
-    hipLaunchKernelGGL(kGenerate, dim3(grid), dim3(BLOCK), 0, s, ps);
+    if (sh.generate) hipLaunchKernelGGL(kGenerate, dim3(grid), dim3(BLOCK), 0, s, ps);
     if (split) {
         hipLaunchKernelGGL(kTagCount, dim3(kTagGrid), dim3(kTagBlock), 0, s, ps);
         hipLaunchKernelGGL(kTagPlan, dim3(1), dim3(1024), 0, s, ps);

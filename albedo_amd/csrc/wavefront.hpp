@@ -171,6 +171,9 @@ struct LaunchShape {
     // samples fit the paths in flight): the free-slot sort is left out, as it
     // would find nothing to do.
     uint32_t free_sort{1};
+    // 0 for the bounces of a render after its first pair when every sample
+    // started in the first bounce: the generation kernel is left out.
+    uint32_t generate{1};
 };
 
 // Enqueues two bounces (generate, extend, bin, shade, shadow, finish, with the
