@@ -197,12 +197,13 @@ __device__ __forceinline__ void storeBlockCounts(const WaveCounts<NQ>& wc, uint3
 // Totals go to totals[q]. One block of 1024 threads: each thread sums a run of
 // consecutive blocks, waves scan their threads' sums with shuffles, and the 16
 // wave totals are combined through LDS.
-template <int NQ>
+// kPer: the most blocks a thread covers, ceil(grid / 1024), fixed at compile time
+// so the loads all go out at once.
+template <int NQ, uint32_t kPer = kMaxGrid / 1024>
 __device__ __forceinline__ void scanBlockCounts(const uint32_t* block_count, uint32_t* block_off, uint32_t grid,
                                                 bool concat, uint32_t* totals)
 {
     constexpr uint32_t kWaves = 1024 / 64;
-    constexpr uint32_t kPer = kMaxGrid / 1024;
     __shared__ uint32_t wsum[kWaves][NQ];
     const uint32_t t = threadIdx.x, lane = __lane_id(), wave = t >> 6;
     const uint32_t per = (grid + 1023) / 1024;
@@ -438,6 +439,7 @@ __global__ __launch_bounds__(BLOCK) void kExtend(PathState ps)
     storeBlockCounts<BLOCK, kBins>(wc, ps.block_count);
 }
 
+template <uint32_t PER>
 __global__ __launch_bounds__(1024) void kPlanBins(PathState ps, uint32_t grid)
 {
     __shared__ uint32_t totals[kBins];
@@ -445,7 +447,7 @@ __global__ __launch_bounds__(1024) void kPlanBins(PathState ps, uint32_t grid)
     // its load overlaps the scan instead of a chain of read-modify-writes after it.
     Counters c{};
     if (threadIdx.x == 0) c = *ps.ctr;
-    scanBlockCounts<kBins>(ps.block_count, ps.block_off, grid, true, totals);
+    scanBlockCounts<kBins, PER>(ps.block_count, ps.block_off, grid, true, totals);
     if (threadIdx.x == 0) {
         // The bookkeeping of the last finish and of this bounce's generation, both
         // already used by kGenerate and kExtend (pendingGeneration) and applied
@@ -1562,7 +1564,11 @@ hipError_t bounce(const PathState& ps, const LaunchShape& sh, hipStream_t s)
         launchTraceQueue<false>(ps, sh, s);
         hipLaunchKernelGGL((kCountBins<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     }
-    hipLaunchKernelGGL(kPlanBins, dim3(1), dim3(1024), 0, s, ps, grid);
+    // One block of 1024 threads, each over ceil(grid / 1024) blocks' counts.
+    if (grid <= 1024) hipLaunchKernelGGL(kPlanBins<1>, dim3(1), dim3(1024), 0, s, ps, grid);
+    else if (grid <= 2048) hipLaunchKernelGGL(kPlanBins<2>, dim3(1), dim3(1024), 0, s, ps, grid);
+    else if (grid <= 4096) hipLaunchKernelGGL(kPlanBins<4>, dim3(1), dim3(1024), 0, s, ps, grid);
+    else hipLaunchKernelGGL(kPlanBins<kMaxGrid / 1024>, dim3(1), dim3(1024), 0, s, ps, grid);
     hipLaunchKernelGGL((kScatterBins<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     hipLaunchKernelGGL((kShade<BLOCK>), dim3(grid), dim3(BLOCK), 0, s, ps);
     if (split) {
