@@ -557,6 +557,16 @@ ab_status Engine::renderBatch(const std::vector<JobSpec>& specs, const std::vect
         shape.pstack = stackClass(plain_depth);
         settleBlocks(shape, cfg, capacity);
     }
+    // A render whose samples all fit never has more paths than samples: by
+    // default the grid-stride kernels then get about four paths per thread at
+    // most (a power of two of blocks, so few graph shapes arise), which shortens
+    // the per-bounce scans.
+    if (shape.free_sort == 0 && cfg.grid_blocks == 0) {
+        const uint64_t want = (total + 4ull * shape.block - 1) / (4ull * shape.block);
+        uint32_t g = 64;
+        while (g < want && g < shape.grid) g *= 2;
+        shape.grid = std::min(g, shape.grid);
+    }
     HIP_TRY(hipMemcpyAsync(d_jobs_, jobs_.data(), sizeof(DevJob) * jobs_.size(), hipMemcpyHostToDevice, stream_),
             "hipMemcpy(jobs)");
     // One segment per job, in job order (the ring holds at least max_jobs entries).
